@@ -1,0 +1,155 @@
+/*
+ * cmdg.h -- C ABI of the MI355X DG right-hand side and LSRK54 stepper for the
+ * dry AtmosModel balance law (spectral-element DG, GLL nodes, hexahedra).
+ *
+ * Every entry point returns an int status (CMDG_OK == 0) and never throws.
+ * Arrays use the reference's column-major storage:
+ *   state       (Np, nstate, nelem)    node n = i + Nq*(j + Nq*k), 0-based i,j,k
+ *   vgeo        (Np, nvgeo, nelem)     field order xi1x1,xi2x1,xi3x1,xi1x2,...,
+ *                                      xi3x3, M, MI, MH, x1, x2, x3, JcV[, x_xi]
+ *   sgeo        (5, Nfp, 6, nelem)     n1, n2, n3, sM, vMI
+ *   vmap-/vmap+ (Nfp, 6, nelem) int64  1-based linear volume dof indices
+ *   elemtobndy  (6, nelem) int64       0 = connected face, else boundary tag
+ *   D           (Nq, Nq)               D[j + Nq*k] = l_k'(xi_j)
+ * Reference layout sources:
+ *   src/Numerics/Mesh/Grids.jl:76-130 (vgeo/sgeo field ids),
+ *   src/Numerics/Mesh/Grids.jl:559-637 (vmap-/vmap+ construction),
+ *   src/Numerics/Mesh/Elements.jl:60-82 (D),
+ *   src/Arrays/MPIStateArrays.jl:46-172 ((Np, nstate, nelem) storage),
+ *   src/Atmos/Model/AtmosModel.jl:387-497 (variable order).
+ *
+ * Geometry, connectivity and the initial auxiliary state are host pointers read
+ * once by cmdg_create (the context derives its device copy in FP64).  State,
+ * tendency and dQ pointers passed to the stepping entry points are device
+ * pointers (hipMalloc'd, caller-owned); work is enqueued on the context stream.
+ */
+#ifndef CMDG_H
+#define CMDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CMDG_OK 0
+#define CMDG_EINVAL 1
+#define CMDG_EHIP 2
+#define CMDG_ENOMEM 3
+#define CMDG_EUNSUPPORTED 4
+
+/* orientation: src/Common/Orientations/Orientations.jl:138-146, 246-257 */
+#define CMDG_NO_ORIENTATION 0
+#define CMDG_FLAT_ORIENTATION 1
+/* reference state: src/Atmos/Model/ref_state.jl:24, 36-60 */
+#define CMDG_NO_REFERENCE_STATE 0
+#define CMDG_HYDROSTATIC_STATE 1
+/* turbulence closure: src/Common/TurbulenceClosures/TurbulenceClosures.jl */
+#define CMDG_CONSTANT_DYNAMIC_VISCOSITY 0 /* :312-404 */
+#define CMDG_CONSTANT_KINEMATIC_VISCOSITY 1
+#define CMDG_SMAGORINSKY_LILLY 2 /* :426-499 */
+#define CMDG_LEARNED_CLOSURE 3   /* per-node MLP eddy viscosity (no reference) */
+/* first-order numerical flux: src/Numerics/DGMethods/NumericalFluxes.jl */
+#define CMDG_RUSANOV 0 /* :219-285 */
+#define CMDG_CENTRAL 1 /* :298-340 */
+/* arithmetic of the tendency kernels */
+#define CMDG_FP64 0
+#define CMDG_MIXED 1 /* FP32 fluxes/derivatives, FP64 state/dQ/reference */
+
+#define CMDG_NSTATE 5 /* rho, rhou1, rhou2, rhou3, rhoe */
+
+typedef struct cmdg_desc {
+    int32_t N;          /* polynomial order (same in all three directions) */
+    int32_t nvgeo;      /* fields per node in vgeo (16 or 25) */
+    int64_t nelem;
+    int32_t naux;       /* auxiliary variables per node in the caller layout */
+    /* positions of the auxiliary fields inside one node's aux vector (-1 = absent)
+     * src/Atmos/Model/AtmosModel.jl:481-497 */
+    int32_t aux_coord;      /* 3 consecutive: x1, x2, x3 */
+    int32_t aux_Phi;        /* orientation.Phi */
+    int32_t aux_gradPhi;    /* 3 consecutive: orientation.gradPhi */
+    int32_t aux_ref_rho;    /* ref_state.rho */
+    int32_t aux_ref_p;      /* ref_state.p */
+    int32_t aux_Delta;      /* turbulence.Delta (Smagorinsky length scale) */
+    int32_t aux_theta_v;    /* moisture.theta_v  (written by every RHS) */
+    int32_t aux_air_T;      /* moisture.air_T    (written by every RHS) */
+    const double *vgeo;
+    const double *sgeo;
+    const int64_t *vmap_minus;
+    const int64_t *vmap_plus;
+    const int64_t *elemtobndy;
+    const double *D;
+    const double *omega;
+    const double *aux_init; /* (Np, naux, nelem) initial auxiliary state */
+    /* thermodynamic and model constants */
+    double R_d, cp_d, cv_d, T_0, grav, MSLP;
+    int32_t orientation;
+    int32_t ref_state;
+    int32_t subtract_off;   /* HydrostaticState(...; subtract_off) */
+    int32_t gravity_source; /* source = (Gravity(),) */
+    int32_t turbulence;
+    int32_t with_divergence;/* ConstantViscosity WithDivergence() */
+    double viscosity;       /* rho*nu (dynamic) or nu (kinematic) */
+    double C_smag;
+    double inv_Pr_turb;
+    int32_t numerical_flux;
+    int32_t precision;
+} cmdg_desc;
+
+typedef struct cmdg_ctx cmdg_ctx;
+
+/* Build a DG context (the DGModel of src/Numerics/DGMethods/DGModel.jl:22-65):
+ * copies geometry/connectivity to the device, takes ownership of a device copy of
+ * the auxiliary state (DGModel.state_auxiliary) and allocates scratch (gradient
+ * flux, ping-pong state).  `stream` is a hipStream_t (NULL = default stream). */
+int cmdg_create(const cmdg_desc *desc, void *stream, cmdg_ctx **out);
+int cmdg_destroy(cmdg_ctx *ctx);
+
+/* tendency = alpha * dQ/dt(Q, t) + beta * tendency
+ * replaces (dg::DGModel)(tendency, Q, _, t, alpha, beta),
+ * src/Numerics/DGMethods/DGModel.jl:85-427; the auxiliary theta_v / air_T
+ * are refreshed from Q as in DGModel.jl:110-116. */
+int cmdg_rhs(cmdg_ctx *ctx, double *tendency, const double *Q, double t,
+             double alpha, double beta);
+
+/* One LSRK54CarpenterKennedy step from t to t + dt:
+ * replaces dostep!(Q, lsrk::LowStorageRungeKutta2N, p, t),
+ * src/Numerics/ODESolvers/LowStorageRungeKuttaMethod.jl:102-144, with the
+ * coefficients of :293-327.  dQ is the solver's low-storage register. */
+int cmdg_lsrk54_step(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt);
+
+/* One low-storage stage: dQ <- RHS(Q, t_stage) + dQ; Q += rkb*dt*dQ;
+ * dQ *= rka_next.  (rhs! call + update! kernel,
+ * LowStorageRungeKuttaMethod.jl:122-142, 146-158) */
+int cmdg_lsrk_stage(cmdg_ctx *ctx, double *Q, double *dQ, double t_stage,
+                    double dt, double rka_next, double rkb);
+
+/* The bare update! kernel on n values (LowStorageRungeKuttaMethod.jl:146-158):
+ * Q[i] += rkb*dt*dQ[i]; dQ[i] *= rka.  `stream` is a hipStream_t. */
+int cmdg_lsrk_update(double *dQ, double *Q, int64_t n, double rka, double rkb,
+                     double dt, void *stream);
+
+/* Device pointer to the context-owned auxiliary state (Np, naux, nelem). */
+int cmdg_aux(cmdg_ctx *ctx, double **aux_device);
+
+/* Device pointer to the gradient flux of the last RHS (Np, ngradflux, nelem),
+ * FP64; ngradflux is 9 (constant viscosity) or 10 (Smagorinsky). */
+int cmdg_gradient_flux(cmdg_ctx *ctx, double **gradflux_device, int32_t *ngradflux);
+
+/* Set the learned-closure MLP: 10 inputs -> 32 -> 32 -> 1 (SiLU), BF16 weights
+ * (row-major [out][in]) and FP32 biases plus FP32 input normalisation. */
+int cmdg_mlp_set_weights(cmdg_ctx *ctx, const uint16_t *w1_bf16, const float *b1,
+                         const uint16_t *w2_bf16, const float *b2,
+                         const uint16_t *w3_bf16, const float *b3,
+                         const float *in_shift, const float *in_scale);
+
+int cmdg_sync(cmdg_ctx *ctx);
+const char *cmdg_last_error(cmdg_ctx *ctx);
+const char *cmdg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CMDG_H */

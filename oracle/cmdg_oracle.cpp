@@ -1,0 +1,701 @@
+// cmdg_oracle.cpp -- CPU oracle for the DG right-hand side + LSRK54 hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  A plain C++ restatement of the reference algorithm,
+// used as the checker of the HIP path and as the CPU baseline; nothing in the
+// product links or calls it.  It keeps the reference's data flow: auxiliary
+// update, volume gradients (horizontal then vertical pass), interface
+// gradients, volume tendency (horizontal pass, then vertical pass with
+// sources), interface tendency with every face flux evaluated from each side,
+// and a separate LSRK update.  Citations are paths in the ClimateMachine.jl tree.
+//
+// Pinned by (tests/test_oracle.py): the isentropic-vortex known-answer errors
+// (test/Numerics/DGMethods/Euler/isentropicvortex.jl:105,106,110,195), LSRK54
+// order 4 (test/Numerics/ODESolvers/ode_tests_convergence.jl:16-42), discrete
+// hydrostatic balance (test/Atmos/Model/discrete_hydrostatic_balance.jl:136),
+// free-stream preservation, conservation, and closed forms for the gradient
+// and Smagorinsky-Lilly closure on linear velocity fields.  Smagorinsky-Lilly
+// has no reference known-answer value; the closed-form test is its only pin.
+#include "../include/cmdg.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr int NQMAX = 8;
+constexpr int NPMAX = NQMAX * NQMAX * NQMAX;
+
+// Julia's eps(x): spacing above |x| (eps(0.0) is the smallest subnormal).
+template <class T> inline T julia_eps(T x) {
+    x = std::fabs(x);
+    return std::nextafter(x, std::numeric_limits<T>::infinity()) - x;
+}
+
+template <class T> struct Ctx {
+    int N, Nq, Np, Nfp, nvgeo, naux;
+    int64_t nelem;
+    cmdg_desc d;  // constants and aux offsets; its pointers are not retained
+    std::vector<T> vgeo, sgeo, D;
+    std::vector<int64_t> vmm, vmp, bndy;
+    std::vector<T> gradflux;  // (Np, ngf, nelem)
+    int ngrad, ngf;
+    bool diffusive;
+    T R_d, cp_d, cv_d, T_0, grav, MSLP, visc, C_smag, inv_Pr;
+};
+
+// vgeo ids (src/Numerics/Mesh/Grids.jl:76-92): xi_a x_d at a + 3 d
+enum { VM = 9, VMI = 10, VX1 = 12 };
+// sgeo ids (Grids.jl:129-130)
+enum { SN1 = 0, SN2, SN3, SSM, SVMI };
+
+template <class T> inline T vg(const Ctx<T> &c, int n, int f, int64_t e) {
+    return c.vgeo[n + (int64_t)c.Np * (f + (int64_t)c.nvgeo * e)];
+}
+template <class T> inline T sg(const Ctx<T> &c, int f, int n, int face, int64_t e) {
+    return c.sgeo[f + 5 * (n + (int64_t)c.Nfp * (face + 6 * e))];
+}
+
+// ------------------------------------------------------------ thermodynamics
+// Thermodynamics.jl 0.5.1 dry relations, reached through
+// src/Atmos/Model/moisture.jl:32-47 (internal_energy) and
+// src/Atmos/Model/thermo_states.jl:67-77 (PhaseDry).
+template <class T> struct TS {
+    T e_int, temp, p;
+};
+
+template <class T> inline T internal_energy(T rho, T rhoe, const T *rhou, T e_pot) {
+    T rinv = T(1) / rho;
+    T rek = rinv * ((rhou[0] * rhou[0] + rhou[1] * rhou[1]) + rhou[2] * rhou[2]) / T(2);
+    T rep = rho * e_pot;
+    T rei = rhoe - rek - rep;
+    return rinv * rei;
+}
+template <class T> inline TS<T> thermo(const Ctx<T> &c, const T *q, T Phi) {
+    TS<T> ts;
+    ts.e_int = internal_energy(q[0], q[4], q + 1, Phi);
+    ts.temp = c.T_0 + ts.e_int / c.cv_d;  // air_temperature
+    ts.p = (c.R_d * q[0]) * ts.temp;      // air_pressure = R_m rho T
+    return ts;
+}
+template <class T> inline T soundspeed(const Ctx<T> &c, const TS<T> &ts) {
+    return std::sqrt(((c.cp_d / c.cv_d) * c.R_d) * ts.temp);
+}
+template <class T> inline T virtual_pottemp(const Ctx<T> &c, const TS<T> &ts) {
+    return T(1) * (ts.temp / std::pow(ts.p / c.MSLP, c.R_d / c.cp_d));
+}
+
+// ------------------------------------------------------------ aux access
+template <class T> struct NodeAux {
+    T Phi, gPhi[3], ref_rho, ref_p, Delta, theta_v;
+};
+template <class T> inline T &auxr(const Ctx<T> &c, T *aux, int n, int s, int64_t e) {
+    return aux[n + (int64_t)c.Np * (s + (int64_t)c.naux * e)];
+}
+template <class T> inline NodeAux<T> load_aux(const Ctx<T> &c, const T *aux, int n, int64_t e) {
+    NodeAux<T> a;
+    auto A = [&](int s) { return aux[n + (int64_t)c.Np * (s + (int64_t)c.naux * e)]; };
+    a.Phi = c.d.aux_Phi >= 0 ? A(c.d.aux_Phi) : -T(0);
+    for (int d = 0; d < 3; ++d) a.gPhi[d] = c.d.aux_gradPhi >= 0 ? A(c.d.aux_gradPhi + d) : T(0);
+    a.ref_rho = c.d.aux_ref_rho >= 0 ? A(c.d.aux_ref_rho) : T(0);
+    a.ref_p = c.d.aux_ref_p >= 0 ? A(c.d.aux_ref_p) : T(0);
+    a.Delta = c.d.aux_Delta >= 0 ? A(c.d.aux_Delta) : T(0);
+    a.theta_v = c.d.aux_theta_v >= 0 ? A(c.d.aux_theta_v) : T(0);
+    return a;
+}
+
+// ------------------------------------------------------------ fluxes
+// F[d][s]: direction d, state s.  Mass: tendencies_mass.jl:7-9; momentum
+// advection + pressure gradient: tendencies_momentum.jl:13-29; energy advection +
+// pressure work: tendencies_energy.jl:7-21; selection atmos_tendencies.jl:16-45.
+template <class T> inline void flux_first_order(const Ctx<T> &c, const T *q, const NodeAux<T> &a, T F[3][5]) {
+    TS<T> ts = thermo(c, q, a.Phi);
+    T p = ts.p;
+    T pmom = (c.d.ref_state == CMDG_HYDROSTATIC_STATE && c.d.subtract_off) ? (p - a.ref_p) : p;
+    T rho = q[0];
+    T u[3] = {q[1] / rho, q[2] / rho, q[3] / rho};
+    for (int d = 0; d < 3; ++d) {
+        F[d][0] = q[1 + d];
+        for (int k = 0; k < 3; ++k) F[d][1 + k] = q[1 + d] * u[k] + (d == k ? pmom : T(0));
+        F[d][4] = u[d] * q[4] + u[d] * p;
+    }
+}
+
+// Turbulence tensors: constant viscosity TurbulenceClosures.jl:366-404,
+// Smagorinsky-Lilly :426-499 (strain-rate helpers :240-283).
+// gf = [grad h_tot (3), S (11,21,31,22,32,33), N2].
+template <class T> struct Turb {
+    T Dt[3];
+    T tau[3][3];
+};
+template <class T> inline Turb<T> turbulence(const Ctx<T> &c, const T *q, const T *gf, const NodeAux<T> &a) {
+    Turb<T> tb;
+    const T *S = gf + 3;
+    T Sm[3][3] = {{S[0], S[1], S[2]}, {S[1], S[3], S[4]}, {S[2], S[4], S[5]}};
+    if (c.d.turbulence == CMDG_SMAGORINSKY_LILLY) {
+        T norm2 = ((((S[0] * S[0] + T(2) * (S[1] * S[1])) + T(2) * (S[2] * S[2])) + S[3] * S[3]) +
+                   T(2) * (S[4] * S[4])) + S[5] * S[5];
+        T normS = std::sqrt(T(2) * norm2);
+        T k[3] = {a.gPhi[0] / c.grav, a.gPhi[1] / c.grav, a.gPhi[2] / c.grav};
+        T Ri = gf[9] / (normS * normS + julia_eps(normS));
+        T fb2 = std::sqrt(std::min(std::max(T(1) - Ri * c.inv_Pr, T(0)), T(1)));
+        T CD = c.C_smag * a.Delta;
+        T nu0 = normS * (CD * CD) + T(1e-5);
+        T dotk = (nu0 * k[0] + nu0 * k[1]) + nu0 * k[2];
+        T nu[3];
+        for (int d = 0; d < 3; ++d) {
+            T nuv = k[d] * dotk;
+            T nuh = nu0 - nuv;
+            nu[d] = nuh + nuv * fb2;
+            tb.Dt[d] = nu[d] * c.inv_Pr;
+        }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) tb.tau[i][j] = (T(-2) * nu[i]) * Sm[i][j];
+    } else {
+        T nu = (c.d.turbulence == CMDG_CONSTANT_DYNAMIC_VISCOSITY) ? c.visc / q[0] : c.visc;
+        T Dt = nu * c.inv_Pr;
+        T trS = (Sm[0][0] + Sm[1][1]) + Sm[2][2];
+        for (int d = 0; d < 3; ++d) tb.Dt[d] = Dt;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                T v = (T(-2) * nu) * Sm[i][j];
+                if (c.d.with_divergence && i == j) v = v + ((T(2) * nu) / T(3)) * trS;
+                tb.tau[i][j] = v;
+            }
+    }
+    return tb;
+}
+
+// Second-order flux: ViscousStress (tendencies_momentum.jl:37-42), ViscousFlux
+// and DiffEnthalpyFlux (tendencies_energy.jl:27-32, 51-56).
+template <class T> inline void flux_second_order(const Ctx<T> &c, const T *q, const T *gf, const NodeAux<T> &a, T F[3][5]) {
+    Turb<T> tb = turbulence(c, q, gf, a);
+    T rho = q[0];
+    for (int d = 0; d < 3; ++d) {
+        F[d][0] = T(0);
+        for (int k = 0; k < 3; ++k) F[d][1 + k] = T(0) + tb.tau[d][k] * rho;
+        T vf = (tb.tau[d][0] * q[1] + tb.tau[d][1] * q[2]) + tb.tau[d][2] * q[3];
+        T dh = ((-tb.Dt[d]) * gf[d]) * rho;
+        F[d][4] = vf + dh;
+    }
+}
+
+// G(q) = (u, h_tot[, theta_v]): AtmosModel.jl:622-690, energy.jl:20-31,
+// TurbulenceClosures.jl:446-453.
+template <class T> inline void gradient_argument(const Ctx<T> &c, const T *q, const NodeAux<T> &a, T *G) {
+    T rinv = T(1) / q[0];
+    G[0] = rinv * q[1];
+    G[1] = rinv * q[2];
+    G[2] = rinv * q[3];
+    TS<T> ts = thermo(c, q, a.Phi);
+    T e_tot = q[4] * (T(1) / q[0]);
+    G[3] = e_tot + c.R_d * ts.temp;
+    G[4] = (c.d.turbulence == CMDG_SMAGORINSKY_LILLY) ? a.theta_v : T(0);
+}
+
+// compute_gradient_flux!: energy.jl:50-59, TurbulenceClosures.jl:366-376,
+// 455-470.  gG[d][s] = d G_s / d x_d.
+template <class T> inline void gradient_flux(const Ctx<T> &c, const T gG[3][5], const NodeAux<T> &a, T *gf) {
+    gf[0] = gG[0][3];
+    gf[1] = gG[1][3];
+    gf[2] = gG[2][3];
+    gf[3] = gG[0][0];
+    gf[4] = (gG[1][0] + gG[0][1]) / T(2);
+    gf[5] = (gG[2][0] + gG[0][2]) / T(2);
+    gf[6] = gG[1][1];
+    gf[7] = (gG[2][1] + gG[1][2]) / T(2);
+    gf[8] = gG[2][2];
+    if (c.d.turbulence == CMDG_SMAGORINSKY_LILLY)
+        gf[9] = ((gG[0][4] * a.gPhi[0] + gG[1][4] * a.gPhi[1]) + gG[2][4] * a.gPhi[2]) / a.theta_v;
+}
+
+// AtmosModel.jl:776-796
+template <class T> inline T wavespeed(const Ctx<T> &c, const T *n, const T *q, const NodeAux<T> &a) {
+    T rinv = T(1) / q[0];
+    T u[3] = {rinv * q[1], rinv * q[2], rinv * q[3]};
+    T uN = std::fabs((n[0] * u[0] + n[1] * u[1]) + n[2] * u[2]);
+    return uN + soundspeed(c, thermo(c, q, a.Phi));
+}
+
+// Impenetrable(FreeSlip()) + Insulating() (bc_momentum.jl:24-43, bc_energy.jl:10-17):
+// the first-order flux reflects the normal momentum, the gradient flux removes it.
+template <class T> inline void wall_state(const T *qm, const T *n, bool gradient, T *qp) {
+    for (int s = 0; s < 5; ++s) qp[s] = qm[s];
+    T dot = (qm[1] * n[0] + qm[2] * n[1]) + qm[3] * n[2];
+    T f = gradient ? dot : T(2) * dot;
+    for (int d = 0; d < 3; ++d) qp[1 + d] = qp[1 + d] - f * n[d];
+}
+
+// ------------------------------------------------------------ RHS passes
+template <class T> void update_aux(Ctx<T> &c, const T *Q, T *aux) {
+    const int Np = c.Np;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < c.nelem; ++e)
+        for (int n = 0; n < Np; ++n) {
+            T q[5];
+            for (int s = 0; s < 5; ++s) q[s] = Q[n + (int64_t)Np * (s + 5 * e)];
+            NodeAux<T> a = load_aux(c, aux, n, e);
+            TS<T> ts = thermo(c, q, a.Phi);
+            if (c.d.aux_theta_v >= 0) auxr(c, aux, n, c.d.aux_theta_v, e) = virtual_pottemp(c, ts);
+            if (c.d.aux_air_T >= 0) auxr(c, aux, n, c.d.aux_air_T, e) = ts.temp;
+        }
+}
+
+// DGModel_kernels.jl:934-1130 (horizontal, writes) and 1132-1328 (vertical, increments)
+template <class T> void volume_gradients(Ctx<T> &c, const T *Q, const T *aux) {
+    const int Nq = c.Nq, Np = c.Np, ng = c.ngrad, ngf = c.ngf;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < c.nelem; ++e) {
+        T G[NPMAX][5];
+        NodeAux<T> A[NPMAX];
+        for (int n = 0; n < Np; ++n) {
+            T q[5];
+            for (int s = 0; s < 5; ++s) q[s] = Q[n + (int64_t)Np * (s + 5 * e)];
+            A[n] = load_aux(c, aux, n, e);
+            gradient_argument(c, q, A[n], G[n]);
+        }
+        for (int k = 0; k < Nq; ++k)
+            for (int j = 0; j < Nq; ++j)
+                for (int i = 0; i < Nq; ++i) {
+                    const int n = i + Nq * (j + Nq * k);
+                    T gG[3][5] = {}, gf[10];
+                    for (int s = 0; s < ng; ++s) {
+                        T g1 = 0, g2 = 0;
+                        for (int m = 0; m < Nq; ++m) {
+                            g1 += c.D[i + Nq * m] * G[m + Nq * (j + Nq * k)][s];
+                            g2 += c.D[j + Nq * m] * G[i + Nq * (m + Nq * k)][s];
+                        }
+                        for (int d = 0; d < 3; ++d) {
+                            T v = -T(0);
+                            v += vg(c, n, 0 + 3 * d, e) * g1;
+                            v += vg(c, n, 1 + 3 * d, e) * g2;
+                            gG[d][s] = v;
+                        }
+                    }
+                    gradient_flux(c, gG, A[n], gf);
+                    for (int s = 0; s < ngf; ++s) c.gradflux[n + (int64_t)Np * (s + (int64_t)ngf * e)] = gf[s];
+                    for (int s = 0; s < ng; ++s) {
+                        T g3 = -T(0);
+                        for (int m = 0; m < Nq; ++m) g3 += c.D[k + Nq * m] * G[i + Nq * (j + Nq * m)][s];
+                        for (int d = 0; d < 3; ++d) gG[d][s] = -T(0) + vg(c, n, 2 + 3 * d, e) * g3;
+                    }
+                    gradient_flux(c, gG, A[n], gf);
+                    for (int s = 0; s < ngf; ++s) c.gradflux[n + (int64_t)Np * (s + (int64_t)ngf * e)] += gf[s];
+                }
+    }
+}
+
+struct FaceIdx {
+    int64_t em, ep;
+    int vm, vp;
+    bool wall;
+};
+template <class T> inline FaceIdx face_idx(const Ctx<T> &c, int64_t e, int f, int fn) {
+    FaceIdx r;
+    int64_t idm = c.vmm[fn + (int64_t)c.Nfp * (f + 6 * e)] - 1;
+    int64_t idp = c.vmp[fn + (int64_t)c.Nfp * (f + 6 * e)] - 1;
+    r.em = e;
+    r.ep = idp / c.Np;
+    r.vm = (int)(idm % c.Np);
+    r.vp = (int)(idp % c.Np);
+    r.wall = c.bndy[f + 6 * e] != 0;
+    if (r.wall) { r.ep = r.em; r.vp = r.vm; }
+    return r;
+}
+
+// DGModel_kernels.jl:1365-1651 with CentralNumericalFluxGradient
+// (NumericalFluxes.jl:67-123)
+template <class T> void interface_gradients(Ctx<T> &c, const T *Q, const T *aux) {
+    const int Np = c.Np, Nfp = c.Nfp, ng = c.ngrad, ngf = c.ngf;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < c.nelem; ++e)
+        for (int f = 0; f < 6; ++f)
+            for (int fn = 0; fn < Nfp; ++fn) {
+                T nv[3] = {sg(c, SN1, fn, f, e), sg(c, SN2, fn, f, e), sg(c, SN3, fn, f, e)};
+                T sM = sg(c, SSM, fn, f, e), vMI = sg(c, SVMI, fn, f, e);
+                FaceIdx x = face_idx(c, e, f, fn);
+                T qm[5], qp[5];
+                for (int s = 0; s < 5; ++s) {
+                    qm[s] = Q[x.vm + (int64_t)Np * (s + 5 * x.em)];
+                    qp[s] = Q[x.vp + (int64_t)Np * (s + 5 * x.ep)];
+                }
+                NodeAux<T> am = load_aux(c, aux, x.vm, x.em), ap = load_aux(c, aux, x.vp, x.ep);
+                T Gm[5], Gp[5], Gs[3][5] = {}, nG[3][5] = {};
+                gradient_argument(c, qm, am, Gm);
+                if (!x.wall) {
+                    gradient_argument(c, qp, ap, Gp);
+                    for (int d = 0; d < 3; ++d)
+                        for (int s = 0; s < ng; ++s) Gs[d][s] = nv[d] * (Gp[s] + Gm[s]) / T(2);
+                } else {
+                    // boundary_state! then the moisture aux refresh of the + side
+                    // (boundaryconditions.jl:58-82), then G* = n G(q+).
+                    T qb[5];
+                    wall_state(qm, nv, true, qb);
+                    NodeAux<T> ab = am;
+                    ab.theta_v = virtual_pottemp(c, thermo(c, qb, ab.Phi));
+                    gradient_argument(c, qb, ab, Gp);
+                    for (int d = 0; d < 3; ++d)
+                        for (int s = 0; s < ng; ++s) Gs[d][s] = nv[d] * Gp[s];
+                }
+                for (int d = 0; d < 3; ++d)
+                    for (int s = 0; s < ng; ++s) nG[d][s] = nv[d] * Gm[s];
+                T gfs[10], gfm[10];
+                gradient_flux(c, Gs, am, gfs);
+                gradient_flux(c, nG, am, gfm);
+                for (int s = 0; s < ngf; ++s)
+                    c.gradflux[x.vm + (int64_t)Np * (s + (int64_t)ngf * x.em)] += vMI * sM * (gfs[s] - gfm[s]);
+            }
+}
+
+// DGModel_kernels.jl:64-309 (horizontal pass, writes alpha*T + beta*tendency) and
+// 312-548 (vertical pass with sources, increments).
+template <class T> void volume_tendency(Ctx<T> &c, T *tend, const T *Q, const T *aux, T alpha, T beta) {
+    const int Nq = c.Nq, Np = c.Np, ngf = c.ngf;
+    const T *gflux = c.gradflux.data();
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < c.nelem; ++e) {
+        T Ft[NPMAX][3][5];
+        T src[NPMAX][5];
+        for (int n = 0; n < Np; ++n) {
+            T q[5], gf[10] = {};
+            for (int s = 0; s < 5; ++s) q[s] = Q[n + (int64_t)Np * (s + 5 * e)];
+            NodeAux<T> a = load_aux(c, aux, n, e);
+            T F[3][5], F2[3][5];
+            flux_first_order(c, q, a, F);
+            if (c.diffusive) {
+                for (int s = 0; s < ngf; ++s) gf[s] = gflux[n + (int64_t)Np * (s + (int64_t)ngf * e)];
+                flux_second_order(c, q, gf, a, F2);
+                for (int d = 0; d < 3; ++d)
+                    for (int s = 0; s < 5; ++s) F[d][s] += F2[d][s];
+            }
+            T M = vg(c, n, VM, e);
+            for (int ax = 0; ax < 3; ++ax) {
+                T x1 = vg(c, n, ax, e), x2 = vg(c, n, ax + 3, e), x3 = vg(c, n, ax + 6, e);
+                for (int s = 0; s < 5; ++s) Ft[n][ax][s] = M * ((x1 * F[0][s] + x2 * F[1][s]) + x3 * F[2][s]);
+            }
+            for (int s = 0; s < 5; ++s) src[n][s] = T(0);
+            if (c.d.gravity_source) {
+                // Gravity (tendencies_momentum.jl:60-72)
+                T drho = (c.d.ref_state == CMDG_HYDROSTATIC_STATE && c.d.subtract_off) ? (q[0] - a.ref_rho) : q[0];
+                for (int d = 0; d < 3; ++d) src[n][1 + d] = (-drho) * a.gPhi[d];
+            }
+        }
+        for (int k = 0; k < Nq; ++k)
+            for (int j = 0; j < Nq; ++j)
+                for (int i = 0; i < Nq; ++i) {
+                    const int n = i + Nq * (j + Nq * k);
+                    T MI = vg(c, n, VMI, e);
+                    for (int s = 0; s < 5; ++s) {
+                        T th = T(0);
+                        for (int m = 0; m < Nq; ++m) {
+                            th += MI * c.D[m + Nq * i] * Ft[m + Nq * (j + Nq * k)][0][s];
+                            th += MI * c.D[m + Nq * j] * Ft[i + Nq * (m + Nq * k)][1][s];
+                        }
+                        int64_t idx = n + (int64_t)Np * (s + 5 * e);
+                        T out = (beta != T(0)) ? alpha * th + beta * tend[idx] : alpha * th;
+                        T tv = T(0);
+                        for (int m = 0; m < Nq; ++m) {
+                            tv += MI * c.D[m + Nq * k] * Ft[i + Nq * (j + Nq * m)][2][s];
+                            if (m == k) tv += src[n][s];
+                        }
+                        tend[idx] = alpha * tv + out;
+                    }
+                }
+    }
+}
+
+// Rusanov / central first-order flux (NumericalFluxes.jl:219-340) plus the
+// central second-order flux (:670-715); walls get a zero second-order flux
+// (AtmosBC override, boundaryconditions.jl:110-165 with FreeSlip/Insulating).
+template <class T>
+inline void numerical_flux(const Ctx<T> &c, const T *nv, const T *qm, const NodeAux<T> &am, const T *gfm,
+                           const T *qp, const NodeAux<T> &ap, const T *gfp, bool wall, T *flux) {
+    T Fm[3][5], Fp[3][5];
+    for (int s = 0; s < 5; ++s) flux[s] = -T(0);
+    flux_first_order(c, qm, am, Fm);
+    flux_first_order(c, qp, ap, Fp);
+    T h[3] = {nv[0] / T(2), nv[1] / T(2), nv[2] / T(2)};
+    for (int s = 0; s < 5; ++s)
+        flux[s] += ((Fm[0][s] + Fp[0][s]) * h[0] + (Fm[1][s] + Fp[1][s]) * h[1]) + (Fm[2][s] + Fp[2][s]) * h[2];
+    if (c.d.numerical_flux == CMDG_RUSANOV) {
+        T mw = std::max(wavespeed(c, nv, qm, am), wavespeed(c, nv, qp, ap));
+        for (int s = 0; s < 5; ++s) flux[s] += (mw * (qm[s] - qp[s])) / T(2);
+    }
+    if (c.diffusive && !wall) {
+        flux_second_order(c, qm, gfm, am, Fm);
+        flux_second_order(c, qp, gfp, ap, Fp);
+        for (int s = 0; s < 5; ++s)
+            flux[s] += ((Fm[0][s] + Fp[0][s]) * h[0] + (Fm[1][s] + Fp[1][s]) * h[1]) + (Fm[2][s] + Fp[2][s]) * h[2];
+    }
+}
+
+// DGModel_kernels.jl:588-901
+template <class T> void interface_tendency(Ctx<T> &c, T *tend, const T *Q, const T *aux, T alpha) {
+    const int Np = c.Np, Nfp = c.Nfp, ngf = c.ngf;
+    const T *gflux = c.gradflux.data();
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < c.nelem; ++e)
+        for (int f = 0; f < 6; ++f)
+            for (int fn = 0; fn < Nfp; ++fn) {
+                T nv[3] = {sg(c, SN1, fn, f, e), sg(c, SN2, fn, f, e), sg(c, SN3, fn, f, e)};
+                T sM = sg(c, SSM, fn, f, e), vMI = sg(c, SVMI, fn, f, e);
+                FaceIdx x = face_idx(c, e, f, fn);
+                T qm[5], qp[5], gfm[10] = {}, gfp[10] = {};
+                for (int s = 0; s < 5; ++s) {
+                    qm[s] = Q[x.vm + (int64_t)Np * (s + 5 * x.em)];
+                    qp[s] = Q[x.vp + (int64_t)Np * (s + 5 * x.ep)];
+                }
+                if (c.diffusive)
+                    for (int s = 0; s < ngf; ++s) {
+                        gfm[s] = gflux[x.vm + (int64_t)Np * (s + (int64_t)ngf * x.em)];
+                        gfp[s] = gflux[x.vp + (int64_t)Np * (s + (int64_t)ngf * x.ep)];
+                    }
+                NodeAux<T> am = load_aux(c, aux, x.vm, x.em), ap = load_aux(c, aux, x.vp, x.ep);
+                T flux[5];
+                if (!x.wall) {
+                    numerical_flux(c, nv, qm, am, gfm, qp, ap, gfp, false, flux);
+                } else {
+                    T qb[5];
+                    wall_state(qm, nv, false, qb);
+                    numerical_flux(c, nv, qm, am, gfm, qb, am, gfm, true, flux);
+                }
+                for (int s = 0; s < 5; ++s) tend[x.vm + (int64_t)Np * (s + 5 * x.em)] -= alpha * vMI * sM * flux[s];
+            }
+}
+
+// DGModel.jl:85-427 for one rank (no ghost exchange)
+template <class T> void rhs(Ctx<T> &c, T *tend, const T *Q, T *aux, T alpha, T beta) {
+    update_aux(c, Q, aux);
+    if (c.diffusive) {
+        volume_gradients(c, Q, aux);
+        interface_gradients(c, Q, aux);
+    }
+    volume_tendency(c, tend, Q, aux, alpha, beta);
+    interface_tendency(c, tend, Q, aux, alpha);
+}
+
+// ------------------------------------------------------------ LSRK54
+// LowStorageRungeKuttaMethod.jl:293-327; RT(a // b) converts numerator and
+// denominator to RT and divides in RT.
+const double RK_NUM[3][5] = {{0, -567301805773.0, -2404267990393.0, -3550918686646.0, -1275806237668.0},
+                             {1432997174477.0, 5161836677717.0, 1720146321549.0, 3134564353537.0, 2277821191437.0},
+                             {0, 1432997174477.0, 2526269341429.0, 2006345519317.0, 2802321613138.0}};
+const double RK_DEN[3][5] = {{1, 1357537059087.0, 2016746695238.0, 2091501179385.0, 842570457699.0},
+                             {9575080441755.0, 13612068292357.0, 2090206949498.0, 4481467310338.0, 14882151754819.0},
+                             {1, 9575080441755.0, 6820363962896.0, 3224310063776.0, 2924317926251.0}};
+template <class T> inline T rkcoef(int which, int i) { return T(RK_NUM[which][i]) / T(RK_DEN[which][i]); }
+
+template <class T> void lsrk54_step(Ctx<T> &c, T *Q, T *dQ, T *aux, T t, T dt) {
+    const int64_t n = (int64_t)c.Np * 5 * c.nelem;
+    for (int s = 0; s < 5; ++s) {
+        rhs(c, dQ, Q, aux, T(1), T(1));
+        (void)t;
+        T a = rkcoef<T>(0, (s + 1) % 5), b = rkcoef<T>(1, s);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            Q[i] += b * dt * dQ[i];  // update! (LowStorageRungeKuttaMethod.jl:146-158)
+            dQ[i] *= a;
+        }
+    }
+}
+
+// ------------------------------------------------------------ aux initialisation
+// dgsem_auxiliary_field_gradient! (DGModel_kernels.jl:3097-3232): volume-only
+// strong derivative, horizontal pass then vertical pass incremented.
+template <class T> void field_gradient(Ctx<T> &c, T *aux, int in, int out) {
+    const int Nq = c.Nq, Np = c.Np;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < c.nelem; ++e) {
+        T f[NPMAX];
+        for (int n = 0; n < Np; ++n) f[n] = auxr(c, aux, n, in, e);
+        for (int k = 0; k < Nq; ++k)
+            for (int j = 0; j < Nq; ++j)
+                for (int i = 0; i < Nq; ++i) {
+                    const int n = i + Nq * (j + Nq * k);
+                    T g1 = 0, g2 = 0, g3 = -T(0);
+                    for (int m = 0; m < Nq; ++m) {
+                        g1 += c.D[i + Nq * m] * f[m + Nq * (j + Nq * k)];
+                        g2 += c.D[j + Nq * m] * f[i + Nq * (m + Nq * k)];
+                        g3 += c.D[k + Nq * m] * f[i + Nq * (j + Nq * m)];
+                    }
+                    for (int d = 0; d < 3; ++d) {
+                        T h = -T(0);
+                        h += vg(c, n, 0 + 3 * d, e) * g1;
+                        h += vg(c, n, 1 + 3 * d, e) * g2;
+                        T v = -T(0) + vg(c, n, 2 + 3 * d, e) * g3;
+                        auxr(c, aux, n, out + d, e) = h + v;
+                    }
+                }
+    }
+}
+
+// PressureGradientModel (ref_state.jl:188-263): the DG tendency of the flux -p I
+// with central fluxes and the minus state mirrored at walls, i.e. grad p.
+template <class T> void pressure_gradient(Ctx<T> &c, const T *p, T *gp) {
+    const int Nq = c.Nq, Np = c.Np, Nfp = c.Nfp;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < c.nelem; ++e) {
+        T Ft[NPMAX][3][3];
+        for (int n = 0; n < Np; ++n) {
+            T pn = p[n + (int64_t)Np * e];
+            T F[3][3];
+            for (int d = 0; d < 3; ++d)
+                for (int k = 0; k < 3; ++k) F[d][k] = (d == k) ? (-T(0) - pn) : -T(0);
+            T M = vg(c, n, VM, e);
+            for (int ax = 0; ax < 3; ++ax) {
+                T x1 = vg(c, n, ax, e), x2 = vg(c, n, ax + 3, e), x3 = vg(c, n, ax + 6, e);
+                for (int s = 0; s < 3; ++s) Ft[n][ax][s] = M * ((x1 * F[0][s] + x2 * F[1][s]) + x3 * F[2][s]);
+            }
+        }
+        for (int k = 0; k < Nq; ++k)
+            for (int j = 0; j < Nq; ++j)
+                for (int i = 0; i < Nq; ++i) {
+                    const int n = i + Nq * (j + Nq * k);
+                    T MI = vg(c, n, VMI, e);
+                    for (int s = 0; s < 3; ++s) {
+                        T th = T(0), tv = T(0);
+                        for (int m = 0; m < Nq; ++m) {
+                            th += MI * c.D[m + Nq * i] * Ft[m + Nq * (j + Nq * k)][0][s];
+                            th += MI * c.D[m + Nq * j] * Ft[i + Nq * (m + Nq * k)][1][s];
+                            tv += MI * c.D[m + Nq * k] * Ft[i + Nq * (j + Nq * m)][2][s];
+                        }
+                        gp[n + (int64_t)Np * (s + 3 * e)] = tv + th;
+                    }
+                }
+        for (int f = 0; f < 6; ++f)
+            for (int fn = 0; fn < Nfp; ++fn) {
+                T nv[3] = {sg(c, SN1, fn, f, e), sg(c, SN2, fn, f, e), sg(c, SN3, fn, f, e)};
+                T sM = sg(c, SSM, fn, f, e), vMI = sg(c, SVMI, fn, f, e);
+                FaceIdx x = face_idx(c, e, f, fn);
+                T pm = p[x.vm + (int64_t)Np * x.em], pp = p[x.vp + (int64_t)Np * x.ep];
+                for (int s = 0; s < 3; ++s) {
+                    T sum = -T(0);
+                    for (int d = 0; d < 3; ++d) {
+                        T Fm = (d == s) ? (-T(0) - pm) : -T(0);
+                        T Fp = (d == s) ? (-T(0) - pp) : -T(0);
+                        sum += (Fm + Fp) * (nv[d] / T(2));
+                    }
+                    gp[x.vm + (int64_t)Np * (s + 3 * x.em)] -= vMI * sM * (-T(0) + sum);
+                }
+            }
+    }
+}
+
+template <class T> Ctx<T> *make_ctx(const cmdg_desc *d) {
+    Ctx<T> *c = new Ctx<T>();
+    c->d = *d;
+    c->N = d->N;
+    c->Nq = d->N + 1;
+    c->Np = c->Nq * c->Nq * c->Nq;
+    c->Nfp = c->Nq * c->Nq;
+    c->nvgeo = d->nvgeo;
+    c->naux = d->naux;
+    c->nelem = d->nelem;
+    const int64_t E = d->nelem;
+    c->vgeo.assign(d->vgeo, d->vgeo + (int64_t)c->Np * c->nvgeo * E);
+    c->sgeo.assign(d->sgeo, d->sgeo + (int64_t)5 * c->Nfp * 6 * E);
+    c->D.assign(d->D, d->D + c->Nq * c->Nq);
+    c->vmm.assign(d->vmap_minus, d->vmap_minus + (int64_t)c->Nfp * 6 * E);
+    c->vmp.assign(d->vmap_plus, d->vmap_plus + (int64_t)c->Nfp * 6 * E);
+    c->bndy.assign(d->elemtobndy, d->elemtobndy + 6 * E);
+    c->R_d = (T)d->R_d;
+    c->cp_d = (T)d->cp_d;
+    c->cv_d = (T)d->cv_d;
+    c->T_0 = (T)d->T_0;
+    c->grav = (T)d->grav;
+    c->MSLP = (T)d->MSLP;
+    c->visc = (T)d->viscosity;
+    c->C_smag = (T)d->C_smag;
+    c->inv_Pr = (T)d->inv_Pr_turb;
+    c->ngrad = (d->turbulence == CMDG_SMAGORINSKY_LILLY) ? 5 : 4;
+    c->ngf = (d->turbulence == CMDG_SMAGORINSKY_LILLY) ? 10 : 9;
+    c->diffusive = true;
+    c->gradflux.assign((size_t)c->Np * c->ngf * E, T(0));
+    return c;
+}
+
+}  // namespace
+
+// ======================================================================= C API
+extern "C" {
+
+struct oracle_ctx {
+    int use_float;
+    void *impl;
+};
+
+oracle_ctx *oracle_create(const cmdg_desc *d, int use_float) {
+    if (!d || d->N < 1 || d->N + 1 > NQMAX || d->nelem <= 0) return nullptr;
+    oracle_ctx *o = new oracle_ctx();
+    o->use_float = use_float;
+    if (use_float) o->impl = make_ctx<float>(d);
+    else o->impl = make_ctx<double>(d);
+    return o;
+}
+
+void oracle_destroy(oracle_ctx *o) {
+    if (!o) return;
+    if (o->use_float) delete (Ctx<float> *)o->impl;
+    else delete (Ctx<double> *)o->impl;
+    delete o;
+}
+
+// tendency = alpha * RHS(Q) + beta * tendency; aux theta_v/air_T refreshed.
+int oracle_rhs(oracle_ctx *o, void *tend, const void *Q, void *aux, double alpha, double beta) {
+    if (o->use_float)
+        rhs(*(Ctx<float> *)o->impl, (float *)tend, (const float *)Q, (float *)aux, (float)alpha, (float)beta);
+    else
+        rhs(*(Ctx<double> *)o->impl, (double *)tend, (const double *)Q, (double *)aux, alpha, beta);
+    return 0;
+}
+
+int oracle_lsrk54_step(oracle_ctx *o, void *Q, void *dQ, void *aux, double t, double dt) {
+    if (o->use_float)
+        lsrk54_step(*(Ctx<float> *)o->impl, (float *)Q, (float *)dQ, (float *)aux, (float)t, (float)dt);
+    else
+        lsrk54_step(*(Ctx<double> *)o->impl, (double *)Q, (double *)dQ, (double *)aux, t, dt);
+    return 0;
+}
+
+int oracle_gradient_flux(oracle_ctx *o, double *out) {
+    if (o->use_float) return 1;
+    Ctx<double> &c = *(Ctx<double> *)o->impl;
+    std::memcpy(out, c.gradflux.data(), c.gradflux.size() * sizeof(double));
+    return 0;
+}
+
+// aux[:, out:out+3, :] = volume derivative of aux[:, in, :]   (double only)
+int oracle_field_gradient(oracle_ctx *o, double *aux, int in, int out) {
+    if (o->use_float) return 1;
+    field_gradient(*(Ctx<double> *)o->impl, aux, in, out);
+    return 0;
+}
+
+// gp (Np, 3, nelem) = DG gradient of p (Np, nelem)   (double only)
+int oracle_pressure_gradient(oracle_ctx *o, const double *p, double *gp) {
+    if (o->use_float) return 1;
+    pressure_gradient(*(Ctx<double> *)o->impl, p, gp);
+    return 0;
+}
+
+// LSRK54 coefficient k of table which (0 = A, 1 = B, 2 = C) in precision prec.
+double oracle_lsrk54_coefficient(int which, int k, int use_float) {
+    return use_float ? (double)rkcoef<float>(which, k) : rkcoef<double>(which, k);
+}
+
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+"""ctypes front end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only the tests, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` use this module, and only as the checker / CPU baseline.  The
+oracle (oracle/cmdg_oracle.cpp) is a plain restatement of the reference's DG
+right-hand side and LSRK54 stepper; see that file's header for what pins it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "climatemachine.jl_amd"))
+
+from cmdg import abi  # noqa: E402  (descriptor layout shared with the library)
+from cmdg.atmos import AtmosConfig  # noqa: E402
+from cmdg.grid import BrickGrid  # noqa: E402
+
+LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        lib = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        lib.oracle_create.argtypes = [C.POINTER(abi.CmdgDesc), C.c_int]
+        lib.oracle_create.restype = vp
+        lib.oracle_destroy.argtypes = [vp]
+        lib.oracle_rhs.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
+        lib.oracle_lsrk54_step.argtypes = [vp, vp, vp, vp, C.c_double, C.c_double]
+        lib.oracle_gradient_flux.argtypes = [vp, vp]
+        lib.oracle_field_gradient.argtypes = [vp, vp, C.c_int, C.c_int]
+        lib.oracle_pressure_gradient.argtypes = [vp, vp, vp]
+        lib.oracle_lsrk54_coefficient.argtypes = [C.c_int, C.c_int, C.c_int]
+        lib.oracle_lsrk54_coefficient.restype = C.c_double
+        lib.oracle_num_threads.restype = C.c_int
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["F_CONTIGUOUS"] or a.ndim == 1
+    return C.c_void_p(a.ctypes.data)
+
+
+class Oracle:
+    """Oracle DG model on ``grid`` with the options of ``cfg``."""
+
+    def __init__(self, cfg: AtmosConfig, grid: BrickGrid, aux: np.ndarray, use_float: bool = False):
+        self.lib = load()
+        self.cfg, self.grid = cfg, grid
+        self.dtype = np.float32 if use_float else np.float64
+        desc, keep = cfg.descriptor(grid, np.asfortranarray(aux, dtype=np.float64))
+        self._ctx = self.lib.oracle_create(C.byref(desc), int(use_float))
+        if not self._ctx:
+            raise RuntimeError("oracle_create failed")
+        self.aux = np.asfortranarray(aux, dtype=self.dtype).copy(order="F")
+
+    def close(self):
+        if self._ctx:
+            self.lib.oracle_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arr(self, a):
+        assert a.dtype == self.dtype and a.flags["F_CONTIGUOUS"], (a.dtype, a.flags)
+        return _p(a)
+
+    def rhs(self, tend: np.ndarray, Q: np.ndarray, alpha: float = 1.0, beta: float = 0.0):
+        self.lib.oracle_rhs(self._ctx, self._arr(tend), self._arr(Q), self._arr(self.aux), alpha, beta)
+
+    def tendency(self, Q: np.ndarray) -> np.ndarray:
+        out = np.zeros_like(Q, order="F")
+        self.rhs(out, np.asfortranarray(Q, dtype=self.dtype))
+        return out
+
+    def lsrk54_step(self, Q: np.ndarray, dQ: np.ndarray, t: float, dt: float):
+        self.lib.oracle_lsrk54_step(self._ctx, self._arr(Q), self._arr(dQ), self._arr(self.aux), t, dt)
+
+    def gradient_flux(self) -> np.ndarray:
+        out = np.zeros((self.grid.Np, self.cfg.ngradflux, self.grid.nelem), order="F")
+        self.lib.oracle_gradient_flux(self._ctx, _p(out))
+        return out
+
+    def field_gradient(self, f: np.ndarray) -> np.ndarray:
+        """DG volume gradient of a nodal field f (Np, E) -> (Np, 3, E)."""
+        tmp = np.zeros((self.grid.Np, self.cfg.naux, self.grid.nelem), order="F")
+        tmp[:, 0, :] = f
+        assert self.cfg.naux >= 4
+        self.lib.oracle_field_gradient(self._ctx, _p(tmp), 0, 1)
+        return np.asfortranarray(tmp[:, 1:4, :])
+
+    def pressure_gradient(self, p: np.ndarray) -> np.ndarray:
+        out = np.zeros((self.grid.Np, 3, self.grid.nelem), order="F")
+        self.lib.oracle_pressure_gradient(self._ctx, _p(np.asfortranarray(p)), _p(out))
+        return out
+
+
+def lsrk54_coefficients(use_float: bool = False):
+    lib = load()
+    return [[lib.oracle_lsrk54_coefficient(w, k, int(use_float)) for k in range(5)] for w in range(3)]
+
+
+def solve(orc: Oracle, Q: np.ndarray, dt: float, timeend: float, t0: float = 0.0):
+    """ODESolvers.solve! loop (ODESolvers.jl:110-158) around the oracle step."""
+    dQ = np.zeros_like(Q, order="F")
+    dtype = orc.dtype
+    t = dtype(t0)
+    dt = dtype(dt)
+    timeend = dtype(timeend)
+    while t < timeend:
+        step_dt = dt
+        final = False
+        if t + dt > timeend:
+            step_dt = dtype(timeend - t)
+            final = True
+        orc.lsrk54_step(Q, dQ, float(t), float(step_dt))
+        t = timeend if final else dtype(t + dt)
+    return float(t)
+
+
+def euclidean_distance(A: np.ndarray, B: np.ndarray, M: np.ndarray) -> float:
+    """MPIStateArrays.euclidean_distance: sqrt(sum (A-B)^2 * M) (MPIStateArrays.jl:628-644)."""
+    E = (A.astype(np.float64) - B.astype(np.float64)) ** 2 * M[:, None, :]
+    return float(np.sqrt(E.sum()))
+
+
+def oracle_init_aux(cfg: AtmosConfig, grid: BrickGrid) -> np.ndarray:
+    """Auxiliary state from the oracle's own DG operators (grad Phi, grad p_ref)."""
+    from cmdg import atmos
+    lay, prm = cfg.layout, cfg.params
+    aux = np.zeros((grid.Np, cfg.naux, grid.nelem), order="F")
+    x1, x2, x3 = grid.coords
+    aux[:, lay["coord"]:lay["coord"] + 3, :] = np.stack([x1, x2, x3], axis=1)
+    orc = Oracle(cfg, grid, aux)
+    if cfg.orientation == abi.FLAT_ORIENTATION:
+        Phi = prm.grav * x3
+        aux[:, lay["Phi"], :] = Phi
+        aux[:, lay["gradPhi"]:lay["gradPhi"] + 3, :] = orc.field_gradient(Phi)
+    if cfg.ref_profile is not None:
+        Phi = aux[:, lay["Phi"], :]
+        gPhi = aux[:, lay["gradPhi"]:lay["gradPhi"] + 3, :]
+        Tv, p = cfg.ref_profile(prm, Phi / prm.grav)
+        gp = orc.pressure_gradient(p)
+        k = gPhi / prm.grav
+        rho = -((k[:, 0] * gp[:, 0] + k[:, 1] * gp[:, 1]) + k[:, 2] * gp[:, 2]) / (
+            (k[:, 0] * gPhi[:, 0] + k[:, 1] * gPhi[:, 1]) + k[:, 2] * gPhi[:, 2])
+        T = p / (prm.R_d * rho)
+        e_int = prm.cv_d * (T - prm.T_0)
+        aux[:, lay["ref_rho"], :] = rho
+        aux[:, lay["ref_p"], :] = p
+        aux[:, lay["ref_T"], :] = prm.T_0 + e_int / prm.cv_d
+        aux[:, lay["ref_rhoe"], :] = rho * ((0.0 + Phi) + e_int)
+    if "Delta" in lay:
+        aux[:, lay["Delta"], :] = atmos.lengthscale(grid)
+    orc.close()
+    return aux
