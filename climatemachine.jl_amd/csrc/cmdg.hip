@@ -70,7 +70,7 @@ __device__ __forceinline__ Thermo thermo(const KParams &P, const double q[5], do
     double rek = rinv * ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]) * 0.5;
     double rep = q[0] * Phi;
     t.e_int = rinv * (q[4] - rek - rep);
-    t.temp = P.T_0 + t.e_int * P.inv_cv;
+    t.temp = P.T_0 + t.e_int / P.cv_d;
     t.p = (P.R_d * q[0]) * t.temp;
     return t;
 }
@@ -129,8 +129,7 @@ __device__ __forceinline__ void flux2(const KParams &P, const double q[5], const
         double norm2 = ((((S[0] * S[0] + 2.0 * (S[1] * S[1])) + 2.0 * (S[2] * S[2])) + S[3] * S[3]) +
                         2.0 * (S[4] * S[4])) + S[5] * S[5];
         double normS = sqrt(2.0 * norm2);
-        double ig = 1.0 / P.grav;
-        double k[3] = {a.gPhi[0] * ig, a.gPhi[1] * ig, a.gPhi[2] * ig};
+        double k[3] = {a.gPhi[0] / P.grav, a.gPhi[1] / P.grav, a.gPhi[2] / P.grav};
         double Ri = gf[9] / (normS * normS + ulp_above(normS));
         double fb2 = sqrt(fmin(fmax(1.0 - Ri * P.inv_Pr, 0.0), 1.0));
         double CD = P.C_smag * a.Delta;
@@ -267,25 +266,38 @@ k_gradient(KParams P, KArrays A, const double *__restrict__ Q, double *__restric
 #pragma unroll
     for (int f = 0; f < 9; ++f) xi[f] = vg[Np * f];
 
-    double gG[3][5];
+    // horizontal pass, then vertical pass (DGModel_kernels.jl:1007-1127, 1235-1325):
+    // the gradient-flux transform is applied to each pass and the results summed
+    double gG[3][5], gV[3][5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gG[d][s] = gV[d][s] = 0.0;
 #pragma unroll
     for (int s = 0; s < NG; ++s) {
-        double g1 = 0.0, g2 = 0.0, g3 = 0.0;
+        double g1 = 0.0, g2 = 0.0, g3 = -0.0;
 #pragma unroll
         for (int m = 0; m < Nq; ++m) {
             g1 += sD[i + Nq * m] * sG[s][m + Nq * (j + Nq * k)];
             g2 += sD[j + Nq * m] * sG[s][i + Nq * (m + Nq * k)];
-            g3 += sD[k + Nq * m] * sG[s][i + Nq * (j + Nq * m)];
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) gG[d][s] = (xi[0 + 3 * d] * g1 + xi[1 + 3 * d] * g2) + xi[2 + 3 * d] * g3;
-    }
-    if (!SMAG)
+        for (int m = 0; m < Nq; ++m) g3 += sD[k + Nq * m] * sG[s][i + Nq * (j + Nq * m)];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) gG[d][4] = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            double v = -0.0;
+            v += xi[0 + 3 * d] * g1;
+            v += xi[1 + 3 * d] * g2;
+            gG[d][s] = v;
+            gV[d][s] = -0.0 + xi[2 + 3 * d] * g3;
+        }
+    }
 
-    double gf[10];
+    double gf[10], gfv[10];
     grad_flux<SMAG>(gG, a, thv, gf);
+    grad_flux<SMAG>(gV, a, thv, gfv);
+#pragma unroll
+    for (int s = 0; s < NGF; ++s) gf[s] += gfv[s];
 
     // lifted face jumps: M^-1 Mf (Phi(G*) - Phi(n G-)), CentralNumericalFluxGradient
     double Gm[5];
@@ -405,17 +417,28 @@ k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__r
     __syncthreads();
     if (!act) return;
 
+    // horizontal pass (written as alpha*T + beta*tendency), then vertical pass with
+    // the source added at its own level (DGModel_kernels.jl:281-306, 453-545)
+    double *tb = tend + (int64_t)Np * 5 * e + n;
     double T[5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
-        double acc = 0.0;
+        double th = 0.0;
 #pragma unroll
         for (int m = 0; m < Nq; ++m) {
-            acc += sD[m + Nq * i] * sF[0][s][m + Nq * (j + Nq * k)];
-            acc += sD[m + Nq * j] * sF[1][s][i + Nq * (m + Nq * k)];
-            acc += sD[m + Nq * k] * sF[2][s][i + Nq * (j + Nq * m)];
+            th += MI * sD[m + Nq * i] * sF[0][s][m + Nq * (j + Nq * k)];
+            th += MI * sD[m + Nq * j] * sF[1][s][i + Nq * (m + Nq * k)];
         }
-        T[s] = MI * acc + src[s];
+        double out;
+        if (MODE == 0) out = (beta != 0.0) ? alpha * th + beta * tb[Np * s] : alpha * th;
+        else out = th + tb[Np * s];
+        double tv = 0.0;
+#pragma unroll
+        for (int m = 0; m < Nq; ++m) {
+            tv += MI * sD[m + Nq * k] * sF[2][s][i + Nq * (j + Nq * m)];
+            if (m == k) tv += src[s];
+        }
+        T[s] = (MODE == 0 ? alpha * tv : tv) + out;
     }
 
     // face fluxes of the faces this node lies on (NumericalFluxes.jl:219-340, 670-715)
@@ -428,7 +451,7 @@ k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__r
         const int64_t idx = fnl[c] + (int64_t)Nfp * (f + 6 * e);
         const double *sgp = A.sg + 5 * idx;
         const double nv[3] = {sgp[0], sgp[1], sgp[2]};
-        const double w = sgp[4] * sgp[3];  // vMI * sM
+        const double w = (MODE == 0 ? alpha * sgp[4] : sgp[4]) * sgp[3];  // alpha * vMI * sM
         const bool wall = A.bnd[f + 6 * e] != 0;
         double qp[5], Fp[3][5], Phip, refpp;
         Aux ap;
@@ -454,7 +477,7 @@ k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__r
         double flux[5];
 #pragma unroll
         for (int s = 0; s < 5; ++s)
-            flux[s] = ((F1[0][s] + Fp[0][s]) * h[0] + (F1[1][s] + Fp[1][s]) * h[1]) + (F1[2][s] + Fp[2][s]) * h[2];
+            flux[s] = -0.0 + (((F1[0][s] + Fp[0][s]) * h[0] + (F1[1][s] + Fp[1][s]) * h[1]) + (F1[2][s] + Fp[2][s]) * h[2]);
         if (P.rusanov) {
             const double rim = 1.0 / q[0], rip = 1.0 / qp[0];
             const double uNm = fabs((nv[0] * (rim * q[1]) + nv[1] * (rim * q[2])) + nv[2] * (rim * q[3]));
@@ -480,17 +503,15 @@ k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__r
         for (int s = 0; s < 5; ++s) T[s] -= w * flux[s];
     }
 
-    double *tb = tend + (int64_t)Np * 5 * e + n;
     if (MODE == 0) {
 #pragma unroll
-        for (int s = 0; s < 5; ++s) tb[Np * s] = (beta != 0.0) ? alpha * T[s] + beta * tb[Np * s] : alpha * T[s];
+        for (int s = 0; s < 5; ++s) tb[Np * s] = T[s];
     } else {
         double *qo = Qout + (int64_t)Np * 5 * e + n;
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
-            const double d = T[s] + tb[Np * s];
-            qo[Np * s] = q[s] + bdt * d;
-            tb[Np * s] = rka * d;
+            qo[Np * s] = q[s] + bdt * T[s];
+            tb[Np * s] = T[s] * rka;
         }
     }
 }

@@ -1,0 +1,61 @@
+"""The C ABI: header, ctypes mirror and the built library agree (no GPU calls)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+from cmdg import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cmdg.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(cmdg_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_the_exported_set():
+    assert header_functions() == sorted(abi.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(abi.LIB_PATH), "run __graft_entry__.build() first"
+    out = subprocess.run(["nm", "-D", "--defined-only", abi.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = set(re.findall(r" T (cmdg_\w+)$", out, re.M))
+    missing = set(header_functions()) - exported
+    assert not missing, missing
+
+
+def test_library_loads_and_reports_version():
+    lib = abi.load_library()
+    assert lib.cmdg_version().decode().startswith("cmdg-mi355x")
+
+
+def test_descriptor_layout_matches_header():
+    text = open(HEADER).read()
+    body = text[text.index("typedef struct cmdg_desc {"):text.index("} cmdg_desc;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    decl = []
+    for line in body.splitlines()[1:]:
+        line = line.strip().rstrip(";").replace("const ", "")
+        if not line:
+            continue
+        _, names = line.split(None, 1)
+        decl += [n.strip().lstrip("*").strip() for n in names.split(",")]
+    assert decl == [f[0] for f in abi.CmdgDesc._fields_]
+
+
+def test_create_rejects_bad_descriptors_without_gpu_work():
+    lib = abi.load_library()
+    d = abi.CmdgDesc()
+    d.N = 9  # not compiled
+    ctx = C.c_void_p()
+    rc = lib.cmdg_create(C.byref(d), None, C.byref(ctx))
+    assert rc == abi.CMDG_EUNSUPPORTED
+    assert b"compiled" in lib.cmdg_last_error(ctx)
+    lib.cmdg_destroy(ctx)
+    assert lib.cmdg_create(None, None, C.byref(ctx)) == abi.CMDG_EINVAL
