@@ -140,6 +140,17 @@ class DGModel:
                                              float(t_stage), float(dt), float(rka_next), float(rkb)),
                     "cmdg_lsrk_stage")
 
+    def set_kernel_timing(self, enable: bool):
+        self._check(self.lib.cmdg_set_kernel_timing(self._ctx, int(bool(enable))), "cmdg_set_kernel_timing")
+
+    def kernel_timing(self) -> dict:
+        """{kind: (total_ms, launches)} since the last call (synchronises)."""
+        n = len(abi.KERNEL_KINDS)
+        ms = (C.c_double * n)()
+        cnt = (C.c_int64 * n)()
+        self._check(self.lib.cmdg_kernel_timing(self._ctx, ms, cnt, n), "cmdg_kernel_timing")
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(abi.KERNEL_KINDS)}
+
     def synchronize(self):
         self._check(self.lib.cmdg_sync(self._ctx), "cmdg_sync")
 

@@ -554,6 +554,13 @@ struct cmdg_ctx {
     double *d_vg = nullptr, *d_sg = nullptr, *d_D = nullptr, *d_aux = nullptr, *d_gf = nullptr, *d_Qs = nullptr;
     int32_t *d_nbr = nullptr, *d_bnd = nullptr;
     std::string err;
+    // optional per-kernel event timing (cmdg_set_kernel_timing)
+    bool timing = false;
+    struct Rec {
+        int kind;
+        hipEvent_t start, stop;
+    };
+    std::vector<Rec> recs;
 };
 
 namespace {
@@ -569,6 +576,25 @@ int check_hip(cmdg_ctx *c, hipError_t e, const char *what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// kernel kinds reported by cmdg_kernel_timing
+enum { KIND_GRADIENT = 0, KIND_TENDENCY = 1, KIND_UPDATE = 2, KIND_COPY = 3, NKIND = 4 };
+
+struct TimedLaunch {
+    cmdg_ctx *c;
+    int kind;
+    hipEvent_t stop = nullptr;
+    TimedLaunch(cmdg_ctx *c_, int kind_) : c(c_), kind(kind_) {
+        if (!c->timing) return;
+        hipEvent_t start;
+        if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) { c->timing = false; return; }
+        (void)hipEventRecord(start, c->stream);
+        c->recs.push_back({kind, start, stop});
+    }
+    ~TimedLaunch() {
+        if (stop) (void)hipEventRecord(stop, c->stream);
+    }
+};
+
 template <class T> int upload(cmdg_ctx *c, T **dst, const T *src, size_t count) {
     hipError_t e = hipMalloc((void **)dst, count * sizeof(T));
     if (e != hipSuccess) return check_hip(c, e, "hipMalloc");
@@ -582,6 +608,7 @@ template <class T> int upload(cmdg_ctx *c, T **dst, const T *src, size_t count) 
 template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux) {
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)c->nelem), block(NT);
+    TimedLaunch tl(c, KIND_GRADIENT);
     if (c->smag)
         hipLaunchKernelGGL((k_gradient<Nq, true>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_aux, c->d_gf, Qcopy, write_aux);
     else
@@ -594,6 +621,7 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alph
                          double *Qout, double rka, double bdt, int write_aux) {
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)c->nelem), block(NT);
+    TimedLaunch tl(c, KIND_TENDENCY);
 #define CMDG_TEND(DIFF, SMAG, MODE)                                                                            \
     hipLaunchKernelGGL((k_tendency<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_gf, \
                        c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux)
@@ -635,6 +663,7 @@ int launch_rhs(cmdg_ctx *c, const double *Q, double *tend, double alpha, double 
 }
 
 int launch_copy(cmdg_ctx *c, double *dst, const double *src, int64_t n) {
+    TimedLaunch tl(c, KIND_COPY);
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, c->stream, dst, src, n);
     return check_hip(c, hipGetLastError(), "copy launch");
 }
@@ -758,6 +787,10 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
 
 int cmdg_destroy(cmdg_ctx *c) {
     if (!c) return CMDG_OK;
+    for (auto &r : c->recs) {
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
     (void)hipFree(c->d_vg);
     (void)hipFree(c->d_sg);
     (void)hipFree(c->d_nbr);
@@ -791,7 +824,10 @@ int cmdg_lsrk_stage(cmdg_ctx *c, double *Q, double *dQ, double t_stage, double d
     if (!c || !Q || !dQ) return c ? fail(c, CMDG_EINVAL, "null pointer") : CMDG_EINVAL;
     int rc = launch_rhs(c, Q, dQ, 1.0, 1.0);
     if (rc) return rc;
-    rc = cmdg_lsrk_update(dQ, Q, (int64_t)c->Np * 5 * c->nelem, rka_next, rkb, dt, c->stream);
+    {
+        TimedLaunch tl(c, KIND_UPDATE);
+        rc = cmdg_lsrk_update(dQ, Q, (int64_t)c->Np * 5 * c->nelem, rka_next, rkb, dt, c->stream);
+    }
     return rc ? fail(c, rc, "update launch") : CMDG_OK;
 }
 
@@ -835,6 +871,30 @@ int cmdg_gradient_flux(cmdg_ctx *c, double **gf, int32_t *ngf) {
 int cmdg_mlp_set_weights(cmdg_ctx *c, const uint16_t *, const float *, const uint16_t *, const float *,
                          const uint16_t *, const float *, const float *, const float *) {
     return fail(c, CMDG_EUNSUPPORTED, "learned closure not built");
+}
+
+int cmdg_set_kernel_timing(cmdg_ctx *c, int enable) {
+    if (!c) return CMDG_EINVAL;
+    c->timing = enable != 0;
+    return CMDG_OK;
+}
+
+int cmdg_kernel_timing(cmdg_ctx *c, double *ms_total, int64_t *launches, int32_t nkinds) {
+    if (!c || !ms_total || !launches || nkinds < 1) return CMDG_EINVAL;
+    for (int k = 0; k < nkinds; ++k) { ms_total[k] = 0.0; launches[k] = 0; }
+    int rc = check_hip(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (rc) return rc;
+    for (auto &r : c->recs) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, r.start, r.stop) == hipSuccess && r.kind < nkinds) {
+            ms_total[r.kind] += ms;
+            launches[r.kind] += 1;
+        }
+        (void)hipEventDestroy(r.start);
+        (void)hipEventDestroy(r.stop);
+    }
+    c->recs.clear();
+    return CMDG_OK;
 }
 
 int cmdg_sync(cmdg_ctx *c) {

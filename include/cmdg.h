@@ -144,6 +144,13 @@ int cmdg_mlp_set_weights(cmdg_ctx *ctx, const uint16_t *w1_bf16, const float *b1
                          const uint16_t *w3_bf16, const float *b3,
                          const float *in_shift, const float *in_scale);
 
+/* Per-kernel device timing (diagnostic; no reference counterpart): when enabled,
+ * every kernel the context launches is bracketed by HIP events on the context
+ * stream.  cmdg_kernel_timing synchronises, returns the summed milliseconds and
+ * launch counts per kind (0 gradient, 1 tendency, 2 update, 3 copy) and resets. */
+int cmdg_set_kernel_timing(cmdg_ctx *ctx, int enable);
+int cmdg_kernel_timing(cmdg_ctx *ctx, double *ms_total, int64_t *launches, int32_t nkinds);
+
 int cmdg_sync(cmdg_ctx *ctx);
 const char *cmdg_last_error(cmdg_ctx *ctx);
 const char *cmdg_version(void);

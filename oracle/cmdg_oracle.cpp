@@ -685,6 +685,22 @@ int oracle_pressure_gradient(oracle_ctx *o, const double *p, double *gp) {
     return 0;
 }
 
+// Second-order flux at one node (double contexts only): q (5), gf (ngf),
+// gPhi (3), Delta -> F (3 x 5, F[d*5 + s]).
+int oracle_flux_second_order(oracle_ctx *o, const double *q, const double *gf, const double *gPhi, double Delta,
+                             double *F) {
+    if (o->use_float) return 1;
+    Ctx<double> &c = *(Ctx<double> *)o->impl;
+    NodeAux<double> a{};
+    for (int d = 0; d < 3; ++d) a.gPhi[d] = gPhi[d];
+    a.Delta = Delta;
+    double F2[3][5];
+    flux_second_order(c, q, gf, a, F2);
+    for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 5; ++s) F[d * 5 + s] = F2[d][s];
+    return 0;
+}
+
 // LSRK54 coefficient k of table which (0 = A, 1 = B, 2 = C) in precision prec.
 double oracle_lsrk54_coefficient(int which, int k, int use_float) {
     return use_float ? (double)rkcoef<float>(which, k) : rkcoef<double>(which, k);

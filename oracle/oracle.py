@@ -43,6 +43,7 @@ def load():
         lib.oracle_lsrk54_coefficient.argtypes = [C.c_int, C.c_int, C.c_int]
         lib.oracle_lsrk54_coefficient.restype = C.c_double
         lib.oracle_num_threads.restype = C.c_int
+        lib.oracle_flux_second_order.argtypes = [vp, vp, vp, vp, C.c_double, vp]
         _lib = lib
     return _lib
 
@@ -104,9 +105,19 @@ class Oracle:
         self.lib.oracle_field_gradient(self._ctx, _p(tmp), 0, 1)
         return np.asfortranarray(tmp[:, 1:4, :])
 
+    def flux_second_order(self, q, gf, gPhi=(0.0, 0.0, 0.0), Delta=0.0) -> np.ndarray:
+        """Second-order flux F[d, s] at one node."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        gf = np.ascontiguousarray(gf, dtype=np.float64)
+        gp = np.ascontiguousarray(gPhi, dtype=np.float64)
+        F = np.zeros(15)
+        self.lib.oracle_flux_second_order(self._ctx, _p(q), _p(gf), _p(gp), float(Delta), _p(F))
+        return F.reshape(3, 5)
+
     def pressure_gradient(self, p: np.ndarray) -> np.ndarray:
         out = np.zeros((self.grid.Np, 3, self.grid.nelem), order="F")
-        self.lib.oracle_pressure_gradient(self._ctx, _p(np.asfortranarray(p)), _p(out))
+        pf = np.asfortranarray(p, dtype=np.float64)  # keep the (possibly copied) input alive
+        self.lib.oracle_pressure_gradient(self._ctx, _p(pf), _p(out))
         return out
 
 
