@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--elements", type=int, default=25)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by element "
@@ -91,6 +93,14 @@ def main():
     import numpy as np
     import torch
     from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device
+
+    if args.copy_calibration_mib > 0:
+        n = args.copy_calibration_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
 
     prm, grid, cfg, aux, Q0, dt = build_workload(args.elements)
     dg = DGModel(cfg, grid, aux=aux)

@@ -68,7 +68,7 @@ class CmdgDesc(C.Structure):
 EXPORTED = (
     "cmdg_create", "cmdg_destroy", "cmdg_rhs", "cmdg_lsrk54_step",
     "cmdg_lsrk_stage", "cmdg_lsrk_update", "cmdg_aux", "cmdg_gradient_flux",
-    "cmdg_mlp_set_weights", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
+    "cmdg_mlp_set_weights", "cmdg_geometry_kind", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
     "cmdg_last_error", "cmdg_version",
 )
 KERNEL_KINDS = ("gradient", "tendency", "update", "copy")
@@ -95,6 +95,7 @@ def load_library(path: str = LIB_PATH):
     lib.cmdg_aux.argtypes = [vp, C.POINTER(vp)]
     lib.cmdg_gradient_flux.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_int32)]
     lib.cmdg_mlp_set_weights.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.cmdg_geometry_kind.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.cmdg_set_kernel_timing.argtypes = [vp, C.c_int]
     lib.cmdg_kernel_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
     lib.cmdg_sync.argtypes = [vp]

@@ -140,6 +140,12 @@ class DGModel:
                                              float(t_stage), float(dt), float(rka_next), float(rkb)),
                     "cmdg_lsrk_stage")
 
+    @property
+    def geometry_kind(self) -> str:
+        k = C.c_int32()
+        self._check(self.lib.cmdg_geometry_kind(self._ctx, C.byref(k)), "cmdg_geometry_kind")
+        return "affine" if k.value == 1 else "general"
+
     def set_kernel_timing(self, enable: bool):
         self._check(self.lib.cmdg_set_kernel_timing(self._ctx, int(bool(enable))), "cmdg_set_kernel_timing")
 

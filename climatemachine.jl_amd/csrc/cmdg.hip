@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -32,489 +33,12 @@
 
 #include "../../include/cmdg.h"
 
+#include "cmdg_physics.h"
+#include "cmdg_general.h"
+#include "cmdg_affine.h"
+
 namespace {
-
-constexpr int NVG = 11;  // device geometry fields per node: xi (9), M, MI
-
-struct KParams {
-    double R_d, cp_d, cv_d, T_0, grav, MSLP, visc, C_smag, inv_Pr;
-    double inv_cv, gammaR, kappa;
-    int32_t ref_sub;        // hydrostatic reference pressure subtracted in the momentum flux
-    int32_t ref_src;        // hydrostatic reference density subtracted in the gravity source
-    int32_t gravity;        // Gravity() source present
-    int32_t turbulence;
-    int32_t with_div;
-    int32_t rusanov;
-    int32_t naux;
-    int32_t a_Phi, a_gPhi, a_ref_rho, a_ref_p, a_Delta, a_theta_v, a_air_T;
-};
-
-struct KArrays {
-    const double *vg;    // (Np, NVG, nelem)
-    const double *sg;    // (5, Nfp, 6, nelem)
-    const int32_t *nbr;  // (Nfp, 6, nelem): e+ * Np + vid+ (0-based)
-    const int32_t *bnd;  // (6, nelem)
-    const double *D;     // (Nq, Nq) column-major
-};
-
-// ------------------------------------------------------------------ physics
-// Thermodynamics.jl dry relations as reached from src/Atmos/Model/moisture.jl:32-47
-// and thermo_states.jl:67-77.
-struct Thermo {
-    double e_int, temp, p;
-};
-
-__device__ __forceinline__ Thermo thermo(const KParams &P, const double q[5], double Phi) {
-    Thermo t;
-    double rinv = 1.0 / q[0];
-    double rek = rinv * ((q[1] * q[1] + q[2] * q[2]) + q[3] * q[3]) * 0.5;
-    double rep = q[0] * Phi;
-    t.e_int = rinv * (q[4] - rek - rep);
-    t.temp = P.T_0 + t.e_int / P.cv_d;
-    t.p = (P.R_d * q[0]) * t.temp;
-    return t;
-}
-
-__device__ __forceinline__ double theta_v(const KParams &P, const Thermo &t) {
-    return t.temp / pow(t.p / P.MSLP, P.kappa);
-}
-
-struct Aux {
-    double Phi, gPhi[3], ref_rho, ref_p, Delta;
-};
-
-__device__ __forceinline__ Aux load_aux(const KParams &P, const double *__restrict__ aux, int Np, int n, int64_t e) {
-    Aux a;
-    const double *b = aux + (int64_t)Np * P.naux * e + n;
-    a.Phi = P.a_Phi >= 0 ? b[(int64_t)Np * P.a_Phi] : -0.0;
-    if (P.a_gPhi >= 0) {
-        a.gPhi[0] = b[(int64_t)Np * P.a_gPhi];
-        a.gPhi[1] = b[(int64_t)Np * (P.a_gPhi + 1)];
-        a.gPhi[2] = b[(int64_t)Np * (P.a_gPhi + 2)];
-    } else {
-        a.gPhi[0] = a.gPhi[1] = a.gPhi[2] = 0.0;
-    }
-    a.ref_rho = P.a_ref_rho >= 0 ? b[(int64_t)Np * P.a_ref_rho] : 0.0;
-    a.ref_p = P.a_ref_p >= 0 ? b[(int64_t)Np * P.a_ref_p] : 0.0;
-    a.Delta = P.a_Delta >= 0 ? b[(int64_t)Np * P.a_Delta] : 0.0;
-    return a;
-}
-
-// First-order flux F[d][s] (tendencies_mass.jl:7-9, tendencies_momentum.jl:13-29,
-// tendencies_energy.jl:7-21).
-__device__ __forceinline__ void flux1(const KParams &P, const double q[5], double p, double ref_p, double F[3][5]) {
-    double pm = P.ref_sub ? (p - ref_p) : p;
-    double u[3] = {q[1] / q[0], q[2] / q[0], q[3] / q[0]};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        F[d][0] = q[1 + d];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) F[d][1 + c] = q[1 + d] * u[c] + (d == c ? pm : 0.0);
-        F[d][4] = u[d] * q[4] + u[d] * p;
-    }
-}
-
-// Julia eps(x) for x >= 0
-__device__ __forceinline__ double ulp_above(double x) { return nextafter(x, INFINITY) - x; }
-
-// Second-order flux: turbulence tensors (TurbulenceClosures.jl:366-404, 472-499) then
-// ViscousStress / ViscousFlux / DiffEnthalpyFlux (tendencies_momentum.jl:37-42,
-// tendencies_energy.jl:27-56).  gf = [grad h_tot (3), S (6), N2].
-template <bool SMAG>
-__device__ __forceinline__ void flux2(const KParams &P, const double q[5], const double *gf, const Aux &a, double F[3][5]) {
-    const double *S = gf + 3;
-    double Sm[3][3] = {{S[0], S[1], S[2]}, {S[1], S[3], S[4]}, {S[2], S[4], S[5]}};
-    double tau[3][3], Dt[3];
-    if (SMAG) {
-        double norm2 = ((((S[0] * S[0] + 2.0 * (S[1] * S[1])) + 2.0 * (S[2] * S[2])) + S[3] * S[3]) +
-                        2.0 * (S[4] * S[4])) + S[5] * S[5];
-        double normS = sqrt(2.0 * norm2);
-        double k[3] = {a.gPhi[0] / P.grav, a.gPhi[1] / P.grav, a.gPhi[2] / P.grav};
-        double Ri = gf[9] / (normS * normS + ulp_above(normS));
-        double fb2 = sqrt(fmin(fmax(1.0 - Ri * P.inv_Pr, 0.0), 1.0));
-        double CD = P.C_smag * a.Delta;
-        double nu0 = normS * (CD * CD) + 1e-5;
-        double dotk = (nu0 * k[0] + nu0 * k[1]) + nu0 * k[2];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double nuv = k[d] * dotk;
-            double nud = (nu0 - nuv) + nuv * fb2;
-            Dt[d] = nud * P.inv_Pr;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) tau[d][c] = (-2.0 * nud) * Sm[d][c];
-        }
-    } else {
-        double nu = (P.turbulence == CMDG_CONSTANT_DYNAMIC_VISCOSITY) ? P.visc / q[0] : P.visc;
-        double trS = (Sm[0][0] + Sm[1][1]) + Sm[2][2];
-        double dv = ((2.0 * nu) / 3.0) * trS;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            Dt[d] = nu * P.inv_Pr;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                double v = (-2.0 * nu) * Sm[d][c];
-                if (P.with_div && d == c) v = v + dv;
-                tau[d][c] = v;
-            }
-        }
-    }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        F[d][0] = 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) F[d][1 + c] = tau[d][c] * q[0];
-        double vf = (tau[d][0] * q[1] + tau[d][1] * q[2]) + tau[d][2] * q[3];
-        F[d][4] = vf + ((-Dt[d]) * gf[d]) * q[0];
-    }
-}
-
-// gradient argument G = (u, h_tot[, theta_v]) (AtmosModel.jl:622-690, energy.jl:20-31)
-template <bool SMAG>
-__device__ __forceinline__ void grad_arg(const KParams &P, const double q[5], const Thermo &t, double thv, double G[5]) {
-    double rinv = 1.0 / q[0];
-    G[0] = rinv * q[1];
-    G[1] = rinv * q[2];
-    G[2] = rinv * q[3];
-    G[3] = q[4] * rinv + P.R_d * t.temp;
-    G[4] = SMAG ? thv : 0.0;
-}
-
-// compute_gradient_flux! (energy.jl:50-59, TurbulenceClosures.jl:366-376, 455-470)
-template <bool SMAG>
-__device__ __forceinline__ void grad_flux(const double gG[3][5], const Aux &a, double thv, double *gf) {
-    gf[0] = gG[0][3];
-    gf[1] = gG[1][3];
-    gf[2] = gG[2][3];
-    gf[3] = gG[0][0];
-    gf[4] = (gG[1][0] + gG[0][1]) * 0.5;
-    gf[5] = (gG[2][0] + gG[0][2]) * 0.5;
-    gf[6] = gG[1][1];
-    gf[7] = (gG[2][1] + gG[1][2]) * 0.5;
-    gf[8] = gG[2][2];
-    if (SMAG) gf[9] = ((gG[0][4] * a.gPhi[0] + gG[1][4] * a.gPhi[1]) + gG[2][4] * a.gPhi[2]) / thv;
-}
-
-// Impenetrable(FreeSlip()) wall state (bc_momentum.jl:24-43)
-__device__ __forceinline__ void wall_state(const double q[5], const double nv[3], double f, double qb[5]) {
-    double dot = (q[1] * nv[0] + q[2] * nv[1]) + q[3] * nv[2];
-    qb[0] = q[0];
-    qb[4] = q[4];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) qb[1 + d] = q[1 + d] - (f * dot) * nv[d];
-}
-
-// Which faces hold node (i,j,k): face f and its face-node index fn (Grids.jl:581-588).
-template <int Nq>
-__device__ __forceinline__ int node_faces(int i, int j, int k, int f[3], int fn[3]) {
-    constexpr int N = Nq - 1;
-    int c = 0;
-    if (i == 0) { f[c] = 0; fn[c] = j + Nq * k; ++c; }
-    if (i == N) { f[c] = 1; fn[c] = j + Nq * k; ++c; }
-    if (j == 0) { f[c] = 2; fn[c] = i + Nq * k; ++c; }
-    if (j == N) { f[c] = 3; fn[c] = i + Nq * k; ++c; }
-    if (k == 0) { f[c] = 4; fn[c] = i + Nq * j; ++c; }
-    if (k == N) { f[c] = 5; fn[c] = i + Nq * j; ++c; }
-    return c;
-}
-
-template <int Nq> constexpr int threads_for() { return ((Nq * Nq * Nq + 63) / 64) * 64; }
-
-// ------------------------------------------------------------------ gradient kernel
-template <int Nq, bool SMAG>
-__global__ void __launch_bounds__(threads_for<Nq>())
-k_gradient(KParams P, KArrays A, const double *__restrict__ Q, double *__restrict__ aux,
-           double *__restrict__ gfo, double *__restrict__ Qcopy, int write_aux) {
-    constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
-    constexpr int NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
-    __shared__ double sD[Nq * Nq];
-    __shared__ double sG[NG][Np];
-    const int64_t e = blockIdx.x;
-    const int n = threadIdx.x;
-    if (n < Nq * Nq) sD[n] = A.D[n];
-    const bool act = n < Np;
-    const int i = n % Nq, j = (n / Nq) % Nq, k = n / (Nq * Nq);
-    double q[5], thv = 0.0;
-    Aux a;
-    Thermo t;
-    if (act) {
-        const double *qb = Q + (int64_t)Np * 5 * e + n;
-#pragma unroll
-        for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
-        if (Qcopy) {
-            double *qc = Qcopy + (int64_t)Np * 5 * e + n;
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qc[Np * s] = q[s];
-        }
-        a = load_aux(P, aux, Np, n, e);
-        t = thermo(P, q, a.Phi);
-        thv = theta_v(P, t);
-        if (write_aux) {
-            double *ab = aux + (int64_t)Np * P.naux * e + n;
-            if (P.a_theta_v >= 0) ab[(int64_t)Np * P.a_theta_v] = thv;
-            if (P.a_air_T >= 0) ab[(int64_t)Np * P.a_air_T] = t.temp;
-        }
-        double G[5];
-        grad_arg<SMAG>(P, q, t, thv, G);
-#pragma unroll
-        for (int s = 0; s < NG; ++s) sG[s][n] = G[s];
-    }
-    __syncthreads();
-    if (!act) return;
-
-    const double *vg = A.vg + (int64_t)Np * NVG * e + n;
-    double xi[9];
-#pragma unroll
-    for (int f = 0; f < 9; ++f) xi[f] = vg[Np * f];
-
-    // horizontal pass, then vertical pass (DGModel_kernels.jl:1007-1127, 1235-1325):
-    // the gradient-flux transform is applied to each pass and the results summed
-    double gG[3][5], gV[3][5];
-#pragma unroll
-    for (int s = 0; s < 5; ++s)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) gG[d][s] = gV[d][s] = 0.0;
-#pragma unroll
-    for (int s = 0; s < NG; ++s) {
-        double g1 = 0.0, g2 = 0.0, g3 = -0.0;
-#pragma unroll
-        for (int m = 0; m < Nq; ++m) {
-            g1 += sD[i + Nq * m] * sG[s][m + Nq * (j + Nq * k)];
-            g2 += sD[j + Nq * m] * sG[s][i + Nq * (m + Nq * k)];
-        }
-#pragma unroll
-        for (int m = 0; m < Nq; ++m) g3 += sD[k + Nq * m] * sG[s][i + Nq * (j + Nq * m)];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            double v = -0.0;
-            v += xi[0 + 3 * d] * g1;
-            v += xi[1 + 3 * d] * g2;
-            gG[d][s] = v;
-            gV[d][s] = -0.0 + xi[2 + 3 * d] * g3;
-        }
-    }
-
-    double gf[10], gfv[10];
-    grad_flux<SMAG>(gG, a, thv, gf);
-    grad_flux<SMAG>(gV, a, thv, gfv);
-#pragma unroll
-    for (int s = 0; s < NGF; ++s) gf[s] += gfv[s];
-
-    // lifted face jumps: M^-1 Mf (Phi(G*) - Phi(n G-)), CentralNumericalFluxGradient
-    double Gm[5];
-    grad_arg<SMAG>(P, q, t, thv, Gm);
-    int fl[3], fnl[3];
-    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
-    for (int c = 0; c < nf; ++c) {
-        const int f = fl[c];
-        const int64_t idx = fnl[c] + (int64_t)Nfp * (f + 6 * e);
-        const double *sgp = A.sg + 5 * idx;
-        double nv[3] = {sgp[0], sgp[1], sgp[2]};
-        const double sM = sgp[3], vMI = sgp[4];
-        double Gs[3][5], nG[3][5], Gp[5];
-        if (A.bnd[f + 6 * e] == 0) {
-            const int32_t nb = A.nbr[idx];
-            const int64_t ep = nb / Np;
-            const int vp = nb - (int)ep * Np;
-            double qp[5];
-            const double *qpb = Q + (int64_t)Np * 5 * ep + vp;
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qp[s] = qpb[Np * s];
-            double Php = P.a_Phi >= 0 ? aux[(int64_t)Np * P.naux * ep + vp + (int64_t)Np * P.a_Phi] : -0.0;
-            Thermo tp = thermo(P, qp, Php);
-            double thp = SMAG ? theta_v(P, tp) : 0.0;
-            grad_arg<SMAG>(P, qp, tp, thp, Gp);
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int s = 0; s < 5; ++s) Gs[d][s] = nv[d] * (Gp[s] + Gm[s]) * 0.5;
-        } else {
-            double qb[5];
-            wall_state(q, nv, 1.0, qb);
-            Thermo tb = thermo(P, qb, a.Phi);
-            double thb = SMAG ? theta_v(P, tb) : 0.0;
-            grad_arg<SMAG>(P, qb, tb, thb, Gp);
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-#pragma unroll
-                for (int s = 0; s < 5; ++s) Gs[d][s] = nv[d] * Gp[s];
-        }
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int s = 0; s < 5; ++s) nG[d][s] = nv[d] * Gm[s];
-        double gfs[10], gfm[10];
-        grad_flux<SMAG>(Gs, a, thv, gfs);
-        grad_flux<SMAG>(nG, a, thv, gfm);
-        const double w = vMI * sM;
-#pragma unroll
-        for (int s = 0; s < NGF; ++s) gf[s] += w * (gfs[s] - gfm[s]);
-    }
-    double *go = gfo + (int64_t)Np * NGF * e + n;
-#pragma unroll
-    for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
-}
-
-// ------------------------------------------------------------------ tendency kernel
-// MODE 0: tend = alpha*RHS + beta*tend.   MODE 1: LSRK stage, tend is dQ:
-//   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka.
-template <int Nq, bool DIFF, bool SMAG, int MODE>
-__global__ void __launch_bounds__(threads_for<Nq>())
-k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__restrict__ gfi,
-           double *__restrict__ aux, double *__restrict__ tend, double alpha, double beta,
-           double *__restrict__ Qout, double rka, double bdt, int write_aux) {
-    constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
-    constexpr int NGF = SMAG ? 10 : 9;
-    __shared__ double sD[Nq * Nq];
-    __shared__ double sF[3][5][Np];
-    const int64_t e = blockIdx.x;
-    const int n = threadIdx.x;
-    if (n < Nq * Nq) sD[n] = A.D[n];
-    const bool act = n < Np;
-    const int i = n % Nq, j = (n / Nq) % Nq, k = n / (Nq * Nq);
-
-    double q[5], gf[10], F1[3][5], F2[3][5], src[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    Aux a;
-    Thermo t;
-    double MI = 0.0;
-    if (act) {
-        const double *qb = Q + (int64_t)Np * 5 * e + n;
-#pragma unroll
-        for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
-        a = load_aux(P, aux, Np, n, e);
-        t = thermo(P, q, a.Phi);
-        if (write_aux) {
-            double *ab = aux + (int64_t)Np * P.naux * e + n;
-            if (P.a_theta_v >= 0) ab[(int64_t)Np * P.a_theta_v] = theta_v(P, t);
-            if (P.a_air_T >= 0) ab[(int64_t)Np * P.a_air_T] = t.temp;
-        }
-        flux1(P, q, t.p, a.ref_p, F1);
-        if (DIFF) {
-            const double *gb = gfi + (int64_t)Np * NGF * e + n;
-#pragma unroll
-            for (int s = 0; s < NGF; ++s) gf[s] = gb[Np * s];
-            flux2<SMAG>(P, q, gf, a, F2);
-        }
-        const double *vg = A.vg + (int64_t)Np * NVG * e + n;
-        const double M = vg[Np * 9];
-        MI = vg[Np * 10];
-#pragma unroll
-        for (int ax = 0; ax < 3; ++ax) {
-            const double x1 = vg[Np * ax], x2 = vg[Np * (ax + 3)], x3 = vg[Np * (ax + 6)];
-#pragma unroll
-            for (int s = 0; s < 5; ++s) {
-                double Fx = F1[0][s], Fy = F1[1][s], Fz = F1[2][s];
-                if (DIFF) { Fx += F2[0][s]; Fy += F2[1][s]; Fz += F2[2][s]; }
-                sF[ax][s][n] = M * ((x1 * Fx + x2 * Fy) + x3 * Fz);
-            }
-        }
-        if (P.gravity) {
-            // Gravity() source (tendencies_momentum.jl:60-72)
-            double drho = P.ref_src ? (q[0] - a.ref_rho) : q[0];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) src[1 + d] = (-drho) * a.gPhi[d];
-        }
-    }
-    __syncthreads();
-    if (!act) return;
-
-    // horizontal pass (written as alpha*T + beta*tendency), then vertical pass with
-    // the source added at its own level (DGModel_kernels.jl:281-306, 453-545)
-    double *tb = tend + (int64_t)Np * 5 * e + n;
-    double T[5];
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        double th = 0.0;
-#pragma unroll
-        for (int m = 0; m < Nq; ++m) {
-            th += MI * sD[m + Nq * i] * sF[0][s][m + Nq * (j + Nq * k)];
-            th += MI * sD[m + Nq * j] * sF[1][s][i + Nq * (m + Nq * k)];
-        }
-        double out;
-        if (MODE == 0) out = (beta != 0.0) ? alpha * th + beta * tb[Np * s] : alpha * th;
-        else out = th + tb[Np * s];
-        double tv = 0.0;
-#pragma unroll
-        for (int m = 0; m < Nq; ++m) {
-            tv += MI * sD[m + Nq * k] * sF[2][s][i + Nq * (j + Nq * m)];
-            if (m == k) tv += src[s];
-        }
-        T[s] = (MODE == 0 ? alpha * tv : tv) + out;
-    }
-
-    // face fluxes of the faces this node lies on (NumericalFluxes.jl:219-340, 670-715)
-    int fl[3], fnl[3];
-    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
-    double csm = 0.0;
-    if (nf > 0 && P.rusanov) csm = sqrt(P.gammaR * t.temp);
-    for (int c = 0; c < nf; ++c) {
-        const int f = fl[c];
-        const int64_t idx = fnl[c] + (int64_t)Nfp * (f + 6 * e);
-        const double *sgp = A.sg + 5 * idx;
-        const double nv[3] = {sgp[0], sgp[1], sgp[2]};
-        const double w = (MODE == 0 ? alpha * sgp[4] : sgp[4]) * sgp[3];  // alpha * vMI * sM
-        const bool wall = A.bnd[f + 6 * e] != 0;
-        double qp[5], Fp[3][5], Phip, refpp;
-        Aux ap;
-        if (!wall) {
-            const int32_t nb = A.nbr[idx];
-            const int64_t ep = nb / Np;
-            const int vp = nb - (int)ep * Np;
-            const double *qpb = Q + (int64_t)Np * 5 * ep + vp;
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qp[s] = qpb[Np * s];
-            ap = load_aux(P, aux, Np, vp, ep);
-            Phip = ap.Phi;
-            refpp = ap.ref_p;
-        } else {
-            wall_state(q, nv, 2.0, qp);
-            ap = a;
-            Phip = a.Phi;
-            refpp = a.ref_p;
-        }
-        Thermo tp = thermo(P, qp, Phip);
-        flux1(P, qp, tp.p, refpp, Fp);
-        const double h[3] = {nv[0] * 0.5, nv[1] * 0.5, nv[2] * 0.5};
-        double flux[5];
-#pragma unroll
-        for (int s = 0; s < 5; ++s)
-            flux[s] = -0.0 + (((F1[0][s] + Fp[0][s]) * h[0] + (F1[1][s] + Fp[1][s]) * h[1]) + (F1[2][s] + Fp[2][s]) * h[2]);
-        if (P.rusanov) {
-            const double rim = 1.0 / q[0], rip = 1.0 / qp[0];
-            const double uNm = fabs((nv[0] * (rim * q[1]) + nv[1] * (rim * q[2])) + nv[2] * (rim * q[3]));
-            const double uNp = fabs((nv[0] * (rip * qp[1]) + nv[1] * (rip * qp[2])) + nv[2] * (rip * qp[3]));
-            const double mw = fmax(uNm + csm, uNp + sqrt(P.gammaR * tp.temp));
-#pragma unroll
-            for (int s = 0; s < 5; ++s) flux[s] += (mw * (q[s] - qp[s])) * 0.5;
-        }
-        if (DIFF && !wall) {
-            const int32_t nb = A.nbr[idx];
-            const int64_t ep = nb / Np;
-            const int vp = nb - (int)ep * Np;
-            double gfp[10];
-            const double *gpb = gfi + (int64_t)Np * NGF * ep + vp;
-#pragma unroll
-            for (int s = 0; s < NGF; ++s) gfp[s] = gpb[Np * s];
-            flux2<SMAG>(P, qp, gfp, ap, Fp);
-#pragma unroll
-            for (int s = 0; s < 5; ++s)
-                flux[s] += ((F2[0][s] + Fp[0][s]) * h[0] + (F2[1][s] + Fp[1][s]) * h[1]) + (F2[2][s] + Fp[2][s]) * h[2];
-        }
-#pragma unroll
-        for (int s = 0; s < 5; ++s) T[s] -= w * flux[s];
-    }
-
-    if (MODE == 0) {
-#pragma unroll
-        for (int s = 0; s < 5; ++s) tb[Np * s] = T[s];
-    } else {
-        double *qo = Qout + (int64_t)Np * 5 * e + n;
-#pragma unroll
-        for (int s = 0; s < 5; ++s) {
-            qo[Np * s] = q[s] + bdt * T[s];
-            tb[Np * s] = T[s] * rka;
-        }
-    }
-}
+using namespace cmdg;
 
 // update! (LowStorageRungeKuttaMethod.jl:146-158)
 __global__ void k_lsrk_update(double *__restrict__ dQ, double *__restrict__ Q, int64_t n, double rka, double bdt) {
@@ -553,6 +77,12 @@ struct cmdg_ctx {
     hipStream_t stream;
     double *d_vg = nullptr, *d_sg = nullptr, *d_D = nullptr, *d_aux = nullptr, *d_gf = nullptr, *d_Qs = nullptr;
     int32_t *d_nbr = nullptr, *d_bnd = nullptr;
+    // affine-brick specialisation (see cmdg_affine.h)
+    bool affine = false;
+    AffArrays AA{};
+    double *d_geo = nullptr, *d_W = nullptr, *d_Wf = nullptr;
+    int32_t *d_nbe = nullptr;
+    double *d_qtr[2] = {nullptr, nullptr}, *d_f2tr = nullptr, *d_str = nullptr;  // face traces
     std::string err;
     // optional per-kernel event timing (cmdg_set_kernel_timing)
     bool timing = false;
@@ -605,11 +135,25 @@ template <class T> int upload(cmdg_ctx *c, T **dst, const T *src, size_t count) 
     return CMDG_OK;
 }
 
-template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux) {
+template <int Nq> int launch_trace_impl(cmdg_ctx *c, const double *Q, double *qtr, double *Qcopy) {
+    dim3 grid((unsigned)xcd_grid(c->nelem)), block(threads_for<Nq>());
+    TimedLaunch tl(c, KIND_COPY);
+    hipLaunchKernelGGL((k_trace_aff<Nq>), grid, block, 0, c->stream, c->nelem, Q, qtr, Qcopy);
+    return check_hip(c, hipGetLastError(), "trace launch");
+}
+
+template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, const double *qtr, double *Qcopy, int write_aux) {
     constexpr int NT = threads_for<Nq>();
-    dim3 grid((unsigned)c->nelem), block(NT);
+    dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_GRADIENT);
-    if (c->smag)
+    if (c->affine) {
+        if (c->smag)
+            hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q, qtr,
+                               c->d_aux, c->d_gf, c->d_f2tr, write_aux);
+        else
+            hipLaunchKernelGGL((k_gradient_aff<Nq, false>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q, qtr,
+                               c->d_aux, c->d_gf, c->d_f2tr, write_aux);
+    } else if (c->smag)
         hipLaunchKernelGGL((k_gradient<Nq, true>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_aux, c->d_gf, Qcopy, write_aux);
     else
         hipLaunchKernelGGL((k_gradient<Nq, false>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_aux, c->d_gf, Qcopy, write_aux);
@@ -617,14 +161,21 @@ template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double 
 }
 
 template <int Nq>
-int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta, int mode,
-                         double *Qout, double rka, double bdt, int write_aux) {
+int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double *tend, double alpha, double beta,
+                         int mode, double *Qout, double *qtr_out, double rka, double bdt, int write_aux) {
     constexpr int NT = threads_for<Nq>();
-    dim3 grid((unsigned)c->nelem), block(NT);
+    dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_TENDENCY);
-#define CMDG_TEND(DIFF, SMAG, MODE)                                                                            \
-    hipLaunchKernelGGL((k_tendency<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_gf, \
-                       c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux)
+#define CMDG_TEND(DIFF, SMAG, MODE)                                                                               \
+    do {                                                                                                          \
+        if (c->affine)                                                                                            \
+            hipLaunchKernelGGL((k_tendency_aff<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->AA,    \
+                               c->nelem, Q, qtr, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, qtr_out,  \
+                               rka, bdt, write_aux);                                                              \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_tendency<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->A, Q,      \
+                               c->d_gf, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux);                 \
+    } while (0)
     if (mode == 0) {
         if (!c->diffusive) CMDG_TEND(false, false, 0);
         else if (c->smag) CMDG_TEND(true, true, 0);
@@ -638,34 +189,134 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alph
     return check_hip(c, hipGetLastError(), "tendency launch");
 }
 
-int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux) {
-    switch (c->Nq) {
-    case 4: return launch_gradient_impl<4>(c, Q, Qcopy, write_aux);
-    case 5: return launch_gradient_impl<5>(c, Q, Qcopy, write_aux);
-    default: return fail(c, CMDG_EUNSUPPORTED, "polynomial order not compiled");
+#define CMDG_DISPATCH_NQ(CALL)                                                 \
+    switch (c->Nq) {                                                           \
+    case 4: return CALL(4);                                                    \
+    case 5: return CALL(5);                                                    \
+    default: return fail(c, CMDG_EUNSUPPORTED, "polynomial order not compiled"); \
     }
+
+int launch_trace(cmdg_ctx *c, const double *Q, double *qtr, double *Qcopy) {
+#define CALL(NQ) launch_trace_impl<NQ>(c, Q, qtr, Qcopy)
+    CMDG_DISPATCH_NQ(CALL)
+#undef CALL
 }
 
-int launch_tendency(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta, int mode, double *Qout,
-                    double rka, double bdt, int write_aux) {
-    switch (c->Nq) {
-    case 4: return launch_tendency_impl<4>(c, Q, tend, alpha, beta, mode, Qout, rka, bdt, write_aux);
-    case 5: return launch_tendency_impl<5>(c, Q, tend, alpha, beta, mode, Qout, rka, bdt, write_aux);
-    default: return fail(c, CMDG_EUNSUPPORTED, "polynomial order not compiled");
-    }
+int launch_gradient(cmdg_ctx *c, const double *Q, const double *qtr, double *Qcopy, int write_aux) {
+#define CALL(NQ) launch_gradient_impl<NQ>(c, Q, qtr, Qcopy, write_aux)
+    CMDG_DISPATCH_NQ(CALL)
+#undef CALL
+}
+
+int launch_tendency(cmdg_ctx *c, const double *Q, const double *qtr, double *tend, double alpha, double beta, int mode,
+                    double *Qout, double *qtr_out, double rka, double bdt, int write_aux) {
+#define CALL(NQ) launch_tendency_impl<NQ>(c, Q, qtr, tend, alpha, beta, mode, Qout, qtr_out, rka, bdt, write_aux)
+    CMDG_DISPATCH_NQ(CALL)
+#undef CALL
 }
 
 // dQ/dt into tend (mode 0) with the auxiliary refresh of DGModel.jl:110-116
 int launch_rhs(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta) {
     int rc;
-    if (c->diffusive && (rc = launch_gradient(c, Q, nullptr, 1))) return rc;
-    return launch_tendency(c, Q, tend, alpha, beta, 0, nullptr, 0.0, 0.0, c->diffusive ? 0 : 1);
+    const double *qtr = nullptr;
+    if (c->affine) {
+        if ((rc = launch_trace(c, Q, c->d_qtr[0], nullptr))) return rc;
+        qtr = c->d_qtr[0];
+    }
+    if (c->diffusive && (rc = launch_gradient(c, Q, qtr, nullptr, 1))) return rc;
+    return launch_tendency(c, Q, qtr, tend, alpha, beta, 0, nullptr, nullptr, 0.0, 0.0, c->diffusive ? 0 : 1);
 }
 
 int launch_copy(cmdg_ctx *c, double *dst, const double *src, int64_t n) {
     TimedLaunch tl(c, KIND_COPY);
     hipLaunchKernelGGL(k_copy, dim3(2048), dim3(256), 0, c->stream, dst, src, n);
     return check_hip(c, hipGetLastError(), "copy launch");
+}
+
+// Exact scale c with c * base[i] == vals[i] for all i (NaN if none near vals[0] / base[0]).
+double exact_scale(const double *vals, int64_t vstride, const double *base, int count) {
+    double c0 = vals[0] / base[0];
+    double cands[5] = {c0, std::nextafter(c0, INFINITY), std::nextafter(c0, -INFINITY), 0, 0};
+    cands[3] = std::nextafter(cands[1], INFINITY);
+    cands[4] = std::nextafter(cands[2], -INFINITY);
+    for (double c : cands) {
+        bool ok = true;
+        for (int i = 0; i < count && ok; ++i) ok = (c * base[i] == vals[i * vstride]);
+        if (ok) return c;
+    }
+    return NAN;
+}
+
+int fmask_vid(int Nq, int f, int fn) {
+    const int a = fn % Nq, b = fn / Nq, N = Nq - 1;
+    switch (f) {
+    case 0: return 0 + Nq * (a + Nq * b);
+    case 1: return N + Nq * (a + Nq * b);
+    case 2: return a + Nq * (0 + Nq * b);
+    case 3: return a + Nq * (N + Nq * b);
+    case 4: return a + Nq * (b + Nq * 0);
+    default: return a + Nq * (b + Nq * N);
+    }
+}
+
+// Bitwise test that the mesh consists of axis-aligned affine bricks with conforming
+// neighbours; fills the per-element record used by the affine kernels.
+bool detect_affine(const cmdg_desc *d, int Nq, std::vector<double> &geo, std::vector<int32_t> &nbe,
+                   std::vector<double> &W, std::vector<double> &Wf) {
+    const int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+    const int64_t E = d->nelem;
+    if (!d->omega) return false;
+    const double *w = d->omega;
+    W.resize(Np);
+    Wf.resize(Nfp);
+    for (int n = 0; n < Np; ++n) W[n] = (w[n / (Nq * Nq)] * w[(n / Nq) % Nq]) * w[n % Nq];
+    for (int fn = 0; fn < Nfp; ++fn) Wf[fn] = w[fn / Nq] * w[fn % Nq];
+    geo.assign((size_t)AFF_GEO * E, 0.0);
+    nbe.assign((size_t)6 * E, -1);
+    for (int64_t e = 0; e < E; ++e) {
+        const double *v = d->vgeo + (int64_t)Np * d->nvgeo * e;
+        double *g = &geo[(size_t)AFF_GEO * e];
+        for (int f = 0; f < 9; ++f) {
+            const bool diag = (f == 0 || f == 4 || f == 8);
+            const double first = v[(int64_t)Np * f];
+            for (int n = 0; n < Np; ++n) {
+                const double val = v[(int64_t)Np * f + n];
+                if (diag ? !(val == first && val != 0.0 && std::isfinite(val)) : val != 0.0) return false;
+            }
+            if (diag) g[f / 4] = first;
+        }
+        const double *M = v + (int64_t)Np * 9, *MI = v + (int64_t)Np * 10;
+        const double J = exact_scale(M, 1, W.data(), Np);
+        if (!(J > 0.0)) return false;
+        for (int n = 0; n < Np; ++n)
+            if (MI[n] != 1.0 / M[n]) return false;
+        g[3] = J;
+        for (int f = 0; f < 6; ++f) {
+            const double *sgp = d->sgeo + (int64_t)5 * Nfp * (f + 6 * e);
+            const double nd = (f & 1) ? 1.0 : -1.0;
+            for (int fn = 0; fn < Nfp; ++fn) {
+                for (int c = 0; c < 3; ++c)
+                    if (sgp[5 * fn + c] != (c == f / 2 ? nd : 0.0)) return false;
+                if (sgp[5 * fn + 4] != MI[fmask_vid(Nq, f, fn)]) return false;
+            }
+            const double sJ = exact_scale(sgp + 3, 5, Wf.data(), Nfp);
+            if (!(sJ > 0.0)) return false;
+            g[4 + f] = sJ;
+            if (d->elemtobndy[f + 6 * e] != 0) continue;
+            const int64_t *vp = d->vmap_plus + (int64_t)Nfp * (f + 6 * e);
+            const int64_t e2 = (vp[0] - 1) / Np;
+            for (int fn = 0; fn < Nfp; ++fn)
+                if (vp[fn] - 1 != (int64_t)Np * e2 + fmask_vid(Nq, f ^ 1, fn)) return false;
+            nbe[6 * e + f] = (int32_t)e2;
+        }
+        if (d->aux_Delta >= 0) {
+            const double *Dl = d->aux_init + (int64_t)Np * (d->aux_Delta + (int64_t)d->naux * e);
+            for (int n = 1; n < Np; ++n)
+                if (Dl[n] != Dl[0]) return false;
+            g[10] = Dl[0];
+        }
+    }
+    return true;
 }
 
 }  // namespace
@@ -782,6 +433,41 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     c->A.nbr = c->d_nbr;
     c->A.bnd = c->d_bnd;
     c->A.D = c->d_D;
+    c->A.nelem = E;
+    {
+        std::vector<double> geo, W, Wf;
+        std::vector<int32_t> nbe;
+        const char *env = std::getenv("CMDG_GENERAL_GEOMETRY");
+        const bool force_general = env && env[0] == '1';
+        if (!force_general && detect_affine(d, Nq, geo, nbe, W, Wf)) {
+            c->affine = true;
+            if ((rc = upload(c, &c->d_geo, geo.data(), geo.size()))) return rc;
+            if ((rc = upload(c, &c->d_nbe, nbe.data(), nbe.size()))) return rc;
+            if ((rc = upload(c, &c->d_W, W.data(), W.size()))) return rc;
+            if ((rc = upload(c, &c->d_Wf, Wf.data(), Wf.size()))) return rc;
+            // static face trace: Phi and p_ref at every face node
+            std::vector<double> st((size_t)E * 6 * TR_S * Nfp, 0.0);
+            for (int64_t e = 0; e < E; ++e)
+                for (int f = 0; f < 6; ++f)
+                    for (int fn = 0; fn < Nfp; ++fn) {
+                        const int vid = fmask_vid(Nq, f, fn);
+                        const double *ab = d->aux_init + (int64_t)Np * d->naux * e + vid;
+                        const size_t o = (((size_t)e * 6 + f) * TR_S) * Nfp + fn;
+                        st[o] = P.a_Phi >= 0 ? ab[(int64_t)Np * P.a_Phi] : -0.0;
+                        st[o + Nfp] = P.a_ref_p >= 0 ? ab[(int64_t)Np * P.a_ref_p] : 0.0;
+                    }
+            if ((rc = upload(c, &c->d_str, st.data(), st.size()))) return rc;
+            for (int b = 0; b < 2; ++b)
+                if ((rc = upload<double>(c, &c->d_qtr[b], nullptr, (size_t)E * 6 * TR_Q * Nfp))) return rc;
+            if ((rc = upload<double>(c, &c->d_f2tr, nullptr, (size_t)E * 6 * TR_F2 * Nfp))) return rc;
+            c->AA.geo = c->d_geo;
+            c->AA.nbe = c->d_nbe;
+            c->AA.W = c->d_W;
+            c->AA.Wf = c->d_Wf;
+            c->AA.D = c->d_D;
+            c->AA.str = c->d_str;
+        }
+    }
     return CMDG_OK;
 }
 
@@ -799,6 +485,14 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_aux);
     (void)hipFree(c->d_gf);
     (void)hipFree(c->d_Qs);
+    (void)hipFree(c->d_geo);
+    (void)hipFree(c->d_nbe);
+    (void)hipFree(c->d_W);
+    (void)hipFree(c->d_Wf);
+    (void)hipFree(c->d_qtr[0]);
+    (void)hipFree(c->d_qtr[1]);
+    (void)hipFree(c->d_f2tr);
+    (void)hipFree(c->d_str);
     delete c;
     return CMDG_OK;
 }
@@ -836,21 +530,35 @@ int cmdg_lsrk54_step(cmdg_ctx *c, double *Q, double *dQ, double t, double dt) {
     if (!c || !Q || !dQ) return c ? fail(c, CMDG_EINVAL, "null pointer") : CMDG_EINVAL;
     // Ping-pong between the caller's Q (C) and the context buffer (S) so that the
     // fused stage update never overwrites a state that neighbouring elements still
-    // read: S <- C (done by the first gradient pass, or a copy), then the stages
-    // run S->C, C->S, S->C, C->S, S->C and the result lands in the caller's Q.
+    // read: S <- C first, then the stages run S->C, C->S, S->C, C->S, S->C and the
+    // result lands in the caller's Q.  On affine bricks the state's face trace is
+    // ping-ponged alongside: written from Q by the trace kernel, then by each
+    // stage's epilogue for the next stage.
     double *C = Q, *S = c->d_Qs;
     const int64_t n = (int64_t)c->Np * 5 * c->nelem;
     int rc;
-    if (!c->diffusive && (rc = launch_copy(c, S, C, n))) return rc;
+    int cur = 0;
+    if (c->affine) {
+        if ((rc = launch_trace(c, C, c->d_qtr[0], S))) return rc;
+    } else if (!c->diffusive) {
+        if ((rc = launch_copy(c, S, C, n))) return rc;
+    }
     double *src = S, *dst = C;
     for (int s = 0; s < 5; ++s) {
         const double a = RKA[(s + 1) % 5], bdt = RKB[s] * dt;
         const int wa = (s == 4);  // aux keeps the state of the last stage, as after dostep!
-        if (c->diffusive && (rc = launch_gradient(c, s == 0 ? C : src, s == 0 ? S : nullptr, wa))) return rc;
-        if ((rc = launch_tendency(c, src, dQ, 1.0, 1.0, 1, dst, a, bdt, c->diffusive ? 0 : wa))) return rc;
+        const double *qtr = c->affine ? c->d_qtr[cur] : nullptr;
+        double *qtr_out = (c->affine && s < 4) ? c->d_qtr[cur ^ 1] : nullptr;
+        if (c->diffusive) {
+            // general geometry: the first gradient pass also leaves S <- C
+            double *Qcopy = (!c->affine && s == 0) ? S : nullptr;
+            if ((rc = launch_gradient(c, (!c->affine && s == 0) ? C : src, qtr, Qcopy, wa))) return rc;
+        }
+        if ((rc = launch_tendency(c, src, qtr, dQ, 1.0, 1.0, 1, dst, qtr_out, a, bdt, c->diffusive ? 0 : wa))) return rc;
         double *tmp = src;
         src = dst;
         dst = tmp;
+        cur ^= 1;
     }
     return CMDG_OK;
 }
@@ -871,6 +579,12 @@ int cmdg_gradient_flux(cmdg_ctx *c, double **gf, int32_t *ngf) {
 int cmdg_mlp_set_weights(cmdg_ctx *c, const uint16_t *, const float *, const uint16_t *, const float *,
                          const uint16_t *, const float *, const float *, const float *) {
     return fail(c, CMDG_EUNSUPPORTED, "learned closure not built");
+}
+
+int cmdg_geometry_kind(cmdg_ctx *c, int32_t *kind) {
+    if (!c || !kind) return CMDG_EINVAL;
+    *kind = c->affine ? 1 : 0;
+    return CMDG_OK;
 }
 
 int cmdg_set_kernel_timing(cmdg_ctx *c, int enable) {

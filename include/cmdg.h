@@ -144,6 +144,12 @@ int cmdg_mlp_set_weights(cmdg_ctx *ctx, const uint16_t *w1_bf16, const float *b1
                          const uint16_t *w3_bf16, const float *b3,
                          const float *in_shift, const float *in_scale);
 
+/* Geometry representation the context chose (diagnostic; no reference counterpart):
+ * 1 = axis-aligned affine bricks (per-element metrics, detected bitwise at create),
+ * 0 = general per-node metrics.  Setting CMDG_GENERAL_GEOMETRY=1 in the environment
+ * before cmdg_create forces 0. */
+int cmdg_geometry_kind(cmdg_ctx *ctx, int32_t *kind);
+
 /* Per-kernel device timing (diagnostic; no reference counterpart): when enabled,
  * every kernel the context launches is bracketed by HIP events on the context
  * stream.  cmdg_kernel_timing synchronises, returns the summed milliseconds and

@@ -56,15 +56,30 @@ def _case(name):
 CASES = ["euler_periodic", "euler_central", "viscous_periodic", "les_walls", "les_central", "les_no_subtract"]
 
 
+def _model(cfg, grid, aux, geometry):
+    """DGModel on the affine-brick kernels or, forced, on the general-mesh kernels."""
+    import os
+    from cmdg.dg import DGModel
+    if geometry == "general":
+        os.environ["CMDG_GENERAL_GEOMETRY"] = "1"
+    try:
+        dg = DGModel(cfg, grid, aux=aux)
+    finally:
+        os.environ.pop("CMDG_GENERAL_GEOMETRY", None)
+    assert dg.geometry_kind == geometry
+    return dg
+
+
+@pytest.mark.parametrize("geometry", ["affine", "general"])
 @pytest.mark.parametrize("name", CASES)
-def test_rhs_parity(name):
+def test_rhs_parity(name, geometry):
     import torch
-    from cmdg.dg import DGModel, to_device, to_host
-    print(f"[parity] rhs {name}", flush=True)
+    from cmdg.dg import to_device, to_host
+    print(f"[parity] rhs {name} {geometry}", flush=True)
     cfg, grid, aux, Q0 = _case(name)
     orc = O.Oracle(cfg, grid, aux)
     ref = orc.tendency(Q0)
-    dg = DGModel(cfg, grid, aux=aux)
+    dg = _model(cfg, grid, aux, geometry)
     Q = to_device(Q0)
     dQ = torch.full_like(Q, float("nan"))       # beta = 0 must not read the tendency
     dg(dQ, Q, None, 0.0)
@@ -96,11 +111,12 @@ def test_rhs_parity(name):
     dg.close()
 
 
+@pytest.mark.parametrize("geometry", ["affine", "general"])
 @pytest.mark.parametrize("name", ["euler_periodic", "les_walls"])
-def test_lsrk54_step_parity(name):
+def test_lsrk54_step_parity(name, geometry):
     import torch
-    from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
-    print(f"[parity] lsrk54 {name}", flush=True)
+    from cmdg.dg import LSRK54CarpenterKennedy, solve, to_device, to_host
+    print(f"[parity] lsrk54 {name} {geometry}", flush=True)
     cfg, grid, aux, Q0 = _case(name)
     dt = 1e-3 if name.startswith("euler") else 5e-3
     orc = O.Oracle(cfg, grid, aux)
@@ -108,7 +124,7 @@ def test_lsrk54_step_parity(name):
     dQr = np.zeros_like(Qr, order="F")
     for k in range(3):
         orc.lsrk54_step(Qr, dQr, k * dt, dt)
-    dg = DGModel(cfg, grid, aux=aux)
+    dg = _model(cfg, grid, aux, geometry)
     Q = to_device(Q0)
     solver = LSRK54CarpenterKennedy(dg, Q, dt=dt, t0=0.0)
     solve(Q, solver, numberofsteps=3, timeend=math.inf)
