@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--elements", type=int, default=25)
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -103,7 +104,8 @@ def main():
         del a, b
 
     prm, grid, cfg, aux, Q0, dt = build_workload(args.elements)
-    dg = DGModel(cfg, grid, aux=aux)
+    from cmdg import abi
+    dg = DGModel(cfg, grid, aux=aux, precision=abi.MIXED if args.precision == "mixed" else abi.FP64)
     Q = to_device(Q0)
     solver = LSRK54CarpenterKennedy(dg, Q, dt=dt, t0=0.0)
     print(f"[bench] {grid.nelem} elements, {grid.Np * grid.nelem} nodes, dt {dt:.4g} s", file=sys.stderr, flush=True)

@@ -36,6 +36,7 @@
 #include "cmdg_physics.h"
 #include "cmdg_general.h"
 #include "cmdg_affine.h"
+#include "cmdg_mixed.h"
 
 namespace {
 using namespace cmdg;
@@ -83,6 +84,10 @@ struct cmdg_ctx {
     double *d_geo = nullptr, *d_W = nullptr, *d_Wf = nullptr;
     int32_t *d_nbe = nullptr;
     double *d_qtr[2] = {nullptr, nullptr}, *d_f2tr = nullptr, *d_str = nullptr;  // face traces
+    // mixed precision (FP32 fluxes and gradient flux)
+    bool mixed = false;
+    MixParams MP{};
+    float *d_gf32 = nullptr, *d_f2tr32 = nullptr;
     std::string err;
     // optional per-kernel event timing (cmdg_set_kernel_timing)
     bool timing = false;
@@ -146,7 +151,14 @@ template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, const d
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_GRADIENT);
-    if (c->affine) {
+    if (c->mixed) {
+        if (c->smag)
+            hipLaunchKernelGGL((k_gradient_mix<Nq, true>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem, Q,
+                               qtr, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux);
+        else
+            hipLaunchKernelGGL((k_gradient_mix<Nq, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem, Q,
+                               qtr, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux);
+    } else if (c->affine) {
         if (c->smag)
             hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q, qtr,
                                c->d_aux, c->d_gf, c->d_f2tr, write_aux);
@@ -168,7 +180,11 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double
     TimedLaunch tl(c, KIND_TENDENCY);
 #define CMDG_TEND(DIFF, SMAG, MODE)                                                                               \
     do {                                                                                                          \
-        if (c->affine)                                                                                            \
+        if (c->mixed)                                                                                             \
+            hipLaunchKernelGGL((k_tendency_mix<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->MP,    \
+                               c->AA, c->nelem, Q, qtr, c->d_gf32, c->d_f2tr32, c->d_aux, tend, alpha, beta,      \
+                               Qout, qtr_out, rka, bdt, write_aux);                                               \
+        else if (c->affine)                                                                                       \
             hipLaunchKernelGGL((k_tendency_aff<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->AA,    \
                                c->nelem, Q, qtr, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, qtr_out,  \
                                rka, bdt, write_aux);                                                              \
@@ -338,7 +354,7 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     if (d->nelem <= 0 || d->nvgeo < 11 || d->naux < 0) return fail(c, CMDG_EINVAL, "bad sizes");
     if (!d->vgeo || !d->sgeo || !d->vmap_minus || !d->vmap_plus || !d->elemtobndy || !d->D)
         return fail(c, CMDG_EINVAL, "missing geometry or connectivity");
-    if (d->precision != CMDG_FP64) return fail(c, CMDG_EUNSUPPORTED, "mixed precision not built in this library");
+    if (d->precision != CMDG_FP64 && d->precision != CMDG_MIXED) return fail(c, CMDG_EINVAL, "unknown precision");
     if (d->turbulence == CMDG_LEARNED_CLOSURE) return fail(c, CMDG_EUNSUPPORTED, "learned closure not built");
     if (d->numerical_flux != CMDG_RUSANOV && d->numerical_flux != CMDG_CENTRAL)
         return fail(c, CMDG_EUNSUPPORTED, "numerical flux");
@@ -468,6 +484,28 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
             c->AA.str = c->d_str;
         }
     }
+    if (d->precision == CMDG_MIXED) {
+        if (!c->affine)
+            return fail(c, CMDG_EUNSUPPORTED, "mixed precision runs on axis-aligned affine brick meshes only");
+        c->mixed = true;
+        MixParams &M = c->MP;
+        M.R_d = (float)d->R_d;
+        M.gammaR = (float)P.gammaR;
+        M.kappa = (float)P.kappa;
+        M.MSLP = (float)d->MSLP;
+        M.grav = (float)d->grav;
+        M.C_smag = (float)d->C_smag;
+        M.inv_Pr = (float)d->inv_Pr_turb;
+        M.visc = (float)d->viscosity;
+        M.dyn_visc = d->turbulence == CMDG_CONSTANT_DYNAMIC_VISCOSITY;
+        M.with_div = d->with_divergence;
+        M.ref_sub = P.ref_sub;
+        M.ref_src = P.ref_src;
+        M.gravity = P.gravity;
+        M.rusanov = P.rusanov;
+        if ((rc = upload<float>(c, &c->d_gf32, nullptr, (size_t)Np * c->ngf * E))) return rc;
+        if ((rc = upload<float>(c, &c->d_f2tr32, nullptr, (size_t)E * 6 * TR_F2 * Nfp))) return rc;
+    }
     return CMDG_OK;
 }
 
@@ -493,6 +531,8 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_qtr[1]);
     (void)hipFree(c->d_f2tr);
     (void)hipFree(c->d_str);
+    (void)hipFree(c->d_gf32);
+    (void)hipFree(c->d_f2tr32);
     delete c;
     return CMDG_OK;
 }
@@ -571,6 +611,7 @@ int cmdg_aux(cmdg_ctx *c, double **aux) {
 
 int cmdg_gradient_flux(cmdg_ctx *c, double **gf, int32_t *ngf) {
     if (!c || !gf) return CMDG_EINVAL;
+    if (c->mixed) return fail(c, CMDG_EUNSUPPORTED, "the mixed-precision gradient flux is stored in FP32");
     *gf = c->d_gf;
     if (ngf) *ngf = c->ngf;
     return CMDG_OK;

@@ -1,0 +1,420 @@
+// cmdg_mixed.h -- mixed-precision DG kernels for affine bricks (CMDG_MIXED).
+//
+// Arithmetic split: the state Q, the low-storage register dQ and the stage update
+// stay FP64; the thermodynamic state (internal energy, temperature, pressure) is
+// evaluated in FP64 from the FP64 state, and the hydrostatic-reference
+// subtractions p - p_ref (pressure perturbation in the momentum flux) and
+// rho - rho_ref (gravity source) are formed in FP64 before rounding.  Without a
+// subtracted reference the momentum flux carries p minus an element constant.  Fluxes,
+// the gradient argument, tensor-product derivatives, face fluxes, the
+// Smagorinsky-Lilly closure and the gradient flux are FP32; the gradient flux and
+// the second-order-flux trace are stored in FP32.  The Rusanov jump q- - q+ is
+// formed in FP64.  Metric terms are computed once in FP64 and rounded on use.
+#pragma once
+#include "cmdg_affine.h"
+
+namespace cmdg {
+
+struct MixParams {
+    float R_d, gammaR, kappa, MSLP, grav, C_smag, inv_Pr, visc;
+    int32_t dyn_visc, with_div, ref_sub, ref_src, gravity, rusanov;
+};
+
+__device__ __forceinline__ void flux1_row_f(const float q[5], const float u[3], float p, float pm, int d, float F[5]) {
+#pragma clang fp contract(fast)
+    const float qd = d == 0 ? q[1] : (d == 1 ? q[2] : q[3]);
+    const float ud = d == 0 ? u[0] : (d == 1 ? u[1] : u[2]);
+    F[0] = qd;
+    F[1] = qd * u[0] + (d == 0 ? pm : 0.0f);
+    F[2] = qd * u[1] + (d == 1 ? pm : 0.0f);
+    F[3] = qd * u[2] + (d == 2 ? pm : 0.0f);
+    F[4] = ud * (q[4] + p);
+}
+
+template <bool SMAG>
+__device__ __forceinline__ void viscosity_f(const MixParams &P, float rho, const float *gf, const float gPhi[3],
+                                            float Delta, float nu[3]) {
+#pragma clang fp contract(fast)
+    if (SMAG) {
+        const float *S = gf + 3;
+        const float norm2 = S[0] * S[0] + S[3] * S[3] + S[5] * S[5] + 2.0f * (S[1] * S[1] + S[2] * S[2] + S[4] * S[4]);
+        const float normS = sqrtf(2.0f * norm2);
+        const float ig = 1.0f / P.grav;
+        const float k[3] = {gPhi[0] * ig, gPhi[1] * ig, gPhi[2] * ig};
+        const float Ri = gf[9] / (normS * normS + (nextafterf(normS, INFINITY) - normS));
+        const float fb2 = sqrtf(fminf(fmaxf(1.0f - Ri * P.inv_Pr, 0.0f), 1.0f));
+        const float CD = P.C_smag * Delta;
+        const float nu0 = normS * (CD * CD) + 1e-5f;
+        const float dotk = nu0 * (k[0] + k[1] + k[2]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float nuv = k[d] * dotk;
+            nu[d] = (nu0 - nuv) + nuv * fb2;
+        }
+    } else {
+        const float v = P.dyn_visc ? P.visc / rho : P.visc;
+        nu[0] = nu[1] = nu[2] = v;
+    }
+}
+
+template <bool SMAG>
+__device__ __forceinline__ void flux2_row_f(const MixParams &P, const float q[5], const float *gf, const float nu[3],
+                                            int d, float F[5]) {
+#pragma clang fp contract(fast)
+    const float *S = gf + 3;
+    const float S0 = d == 0 ? S[0] : (d == 1 ? S[1] : S[2]);
+    const float S1 = d == 0 ? S[1] : (d == 1 ? S[3] : S[4]);
+    const float S2 = d == 0 ? S[2] : (d == 1 ? S[4] : S[5]);
+    const float nud = d == 0 ? nu[0] : (d == 1 ? nu[1] : nu[2]);
+    const float m2 = -2.0f * nud;
+    float t0 = m2 * S0, t1 = m2 * S1, t2 = m2 * S2;
+    if (!SMAG && P.with_div) {
+        const float dv = (2.0f * nud / 3.0f) * (S[0] + S[3] + S[5]);
+        if (d == 0) t0 += dv;
+        if (d == 1) t1 += dv;
+        if (d == 2) t2 += dv;
+    }
+    F[0] = 0.0f;
+    F[1] = t0 * q[0];
+    F[2] = t1 * q[0];
+    F[3] = t2 * q[0];
+    const float gd = d == 0 ? gf[0] : (d == 1 ? gf[1] : gf[2]);
+    F[4] = t0 * q[1] + t1 * q[2] + t2 * q[3] - (nud * P.inv_Pr) * gd * q[0];
+}
+
+// gradient flux of the 3 x 5 gradient (energy.jl:50-59, TurbulenceClosures.jl:455-470)
+template <bool SMAG>
+__device__ __forceinline__ void grad_flux_f(const float g[3][5], const float gPhi[3], float thv, float *gf) {
+#pragma clang fp contract(fast)
+    gf[0] = g[0][3];
+    gf[1] = g[1][3];
+    gf[2] = g[2][3];
+    gf[3] = g[0][0];
+    gf[4] = 0.5f * (g[1][0] + g[0][1]);
+    gf[5] = 0.5f * (g[2][0] + g[0][2]);
+    gf[6] = g[1][1];
+    gf[7] = 0.5f * (g[2][1] + g[1][2]);
+    gf[8] = g[2][2];
+    if (SMAG) gf[9] = (g[0][4] * gPhi[0] + g[1][4] * gPhi[1] + g[2][4] * gPhi[2]) / thv;
+}
+
+struct ThermoD {
+    double temp, p;
+};
+__device__ __forceinline__ ThermoD thermo_d(const KParams &P, const double q[5], double Phi) {
+#pragma clang fp contract(fast)
+    const double rinv = 1.0 / q[0];
+    const double e_int = rinv * (q[4] - 0.5 * rinv * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) - q[0] * Phi);
+    ThermoD t;
+    t.temp = P.T_0 + e_int * P.inv_cv;
+    t.p = P.R_d * q[0] * t.temp;
+    return t;
+}
+
+// gradient argument (u, h_tot, theta_v) in FP32 from the FP64 state and thermodynamics
+template <bool SMAG>
+__device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M, const double q[5], const ThermoD &t,
+                                           float G[5]) {
+#pragma clang fp contract(fast)
+    const double rinv = 1.0 / q[0];
+    G[0] = (float)(rinv * q[1]);
+    G[1] = (float)(rinv * q[2]);
+    G[2] = (float)(rinv * q[3]);
+    G[3] = (float)(q[4] * rinv + P.R_d * t.temp);
+    G[4] = SMAG ? (float)t.temp / powf((float)(t.p / P.MSLP), M.kappa) : 0.0f;
+}
+
+// ------------------------------------------------------------------ gradient (mixed)
+template <int Nq, bool SMAG>
+__global__ void __launch_bounds__(threads_for<Nq>())
+k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
+               const double *__restrict__ qtr, double *__restrict__ aux, float *__restrict__ gfo,
+               float *__restrict__ f2tr, int write_aux) {
+#pragma clang fp contract(fast)
+    constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+    constexpr int NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
+    __shared__ float sD[Nq * Nq];
+    __shared__ float sG[NG][Np];
+    const int64_t e = xcd_element(blockIdx.x, nelem);
+    if (e >= nelem) return;
+    const int n = threadIdx.x;
+    if (n < Nq * Nq) sD[n] = (float)A.D[n];
+    const bool act = n < Np;
+    const int i = n % Nq, j = (n / Nq) % Nq, k = n / (Nq * Nq);
+    const double *ge = A.geo + (int64_t)AFF_GEO * e;
+    double q[5];
+    float G[5], gPhi[3] = {0.0f, 0.0f, 0.0f}, thv = 1.0f;
+    double Phi = -0.0;
+    if (act) {
+        const double *qb = Q + (int64_t)Np * 5 * e + n;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
+        const double *ab = aux + (int64_t)Np * P.naux * e + n;
+        if (P.a_Phi >= 0) Phi = ab[(int64_t)Np * P.a_Phi];
+        if (P.a_gPhi >= 0) {
+            gPhi[0] = (float)ab[(int64_t)Np * P.a_gPhi];
+            gPhi[1] = (float)ab[(int64_t)Np * (P.a_gPhi + 1)];
+            gPhi[2] = (float)ab[(int64_t)Np * (P.a_gPhi + 2)];
+        }
+        const ThermoD t = thermo_d(P, q, Phi);
+        grad_arg_f<SMAG>(P, MP, q, t, G);
+        thv = SMAG ? G[4] : 1.0f;
+        if (write_aux) {
+            double *aw = aux + (int64_t)Np * P.naux * e + n;
+            if (P.a_theta_v >= 0) aw[(int64_t)Np * P.a_theta_v] = t.temp / pow(t.p / P.MSLP, P.kappa);
+            if (P.a_air_T >= 0) aw[(int64_t)Np * P.a_air_T] = t.temp;
+        }
+#pragma unroll
+        for (int s = 0; s < NG; ++s) sG[s][n] = G[s];
+    }
+    __syncthreads();
+    if (!act) return;
+
+    const float xi0 = (float)ge[0], xi1 = (float)ge[1], xi2 = (float)ge[2];
+    float g[3][5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+        if (s < NG) {
+#pragma unroll
+            for (int m = 0; m < Nq; ++m) {
+                g1 += sD[i + Nq * m] * sG[s][m + Nq * (j + Nq * k)];
+                g2 += sD[j + Nq * m] * sG[s][i + Nq * (m + Nq * k)];
+                g3 += sD[k + Nq * m] * sG[s][i + Nq * (j + Nq * m)];
+            }
+        }
+        g[0][s] = xi0 * g1;
+        g[1][s] = xi1 * g2;
+        g[2][s] = xi2 * g3;
+    }
+    // lifted central jumps, accumulated into the gradient: M^-1 Mf n (G* - G-)
+    const float MI = (float)(1.0 / (ge[3] * A.W[n]));
+    int fl[3], fnl[3];
+    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
+    for (int c = 0; c < nf; ++c) {
+        const int f = fl[c], d = f >> 1, fn = fnl[c];
+        const float nd = (f & 1) ? 1.0f : -1.0f;
+        const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
+        const int32_t ne = A.nbe[6 * e + f];
+        float Gp[5], jump[5];
+        if (ne >= 0) {
+            const int fo = f ^ 1;
+            double qp[5];
+#pragma unroll
+            for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
+            const double Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+            grad_arg_f<SMAG>(P, MP, qp, thermo_d(P, qp, Php), Gp);
+#pragma unroll
+            for (int s = 0; s < 5; ++s) jump[s] = 0.5f * (Gp[s] - G[s]);
+        } else {
+            double qb[5] = {q[0], q[1], q[2], q[3], q[4]};
+            qb[1 + d] = 0.0;  // the gradient wall state removes the normal momentum
+            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, Phi), Gp);
+#pragma unroll
+            for (int s = 0; s < 5; ++s) jump[s] = Gp[s] - G[s];
+        }
+#pragma unroll
+        for (int s = 0; s < 5; ++s) g[d][s] += (w * nd) * jump[s];
+    }
+    float gf[10];
+    grad_flux_f<SMAG>(g, gPhi, thv, gf);
+    float *go = gfo + (int64_t)Np * NGF * e + n;
+#pragma unroll
+    for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
+
+    if (nf > 0) {
+        float nu[3], qf[5];
+#pragma unroll
+        for (int s = 0; s < 5; ++s) qf[s] = (float)q[s];
+        viscosity_f<SMAG>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
+        for (int c = 0; c < nf; ++c) {
+            float F2[5];
+            flux2_row_f<SMAG>(MP, qf, gf, nu, fl[c] >> 1, F2);
+#pragma unroll
+            for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, fl[c], v, TR_F2, fnl[c])] = F2[1 + v];
+        }
+    }
+}
+
+// ------------------------------------------------------------------ tendency (mixed)
+template <int Nq, bool DIFF, bool SMAG, int MODE>
+__global__ void __launch_bounds__(threads_for<Nq>())
+k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
+               const double *__restrict__ qtr, const float *__restrict__ gfi, const float *__restrict__ f2tr,
+               double *__restrict__ aux, double *__restrict__ tend, double alpha, double beta,
+               double *__restrict__ Qout, double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
+#pragma clang fp contract(fast)
+    constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+    constexpr int NGF = SMAG ? 10 : 9;
+    __shared__ float sD[Nq * Nq];
+    __shared__ float sF[3][5][Np];
+    const int64_t e = xcd_element(blockIdx.x, nelem);
+    if (e >= nelem) return;
+    const int n = threadIdx.x;
+    if (n < Nq * Nq) sD[n] = (float)A.D[n];
+    const bool act = n < Np;
+    const int i = n % Nq, j = (n / Nq) % Nq, k = n / (Nq * Nq);
+    const double *ge = A.geo + (int64_t)AFF_GEO * e;
+
+    double q[5], Phi = -0.0, ref_p = 0.0, p_e = 0.0;
+    float qf[5], uf[3], gf[10], nu[3], gPhi[3] = {0.0f, 0.0f, 0.0f}, pf = 0.0f, pmf = 0.0f, cm = 0.0f, drho = 0.0f;
+    float MI = 0.0f;
+    if (act) {
+        const double *qb = Q + (int64_t)Np * 5 * e + n;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
+        const double *ab = aux + (int64_t)Np * P.naux * e + n;
+        if (P.a_Phi >= 0) Phi = ab[(int64_t)Np * P.a_Phi];
+        if (P.a_ref_p >= 0) ref_p = ab[(int64_t)Np * P.a_ref_p];
+        double ref_rho = P.a_ref_rho >= 0 ? ab[(int64_t)Np * P.a_ref_rho] : 0.0;
+        if (P.a_gPhi >= 0) {
+            gPhi[0] = (float)ab[(int64_t)Np * P.a_gPhi];
+            gPhi[1] = (float)ab[(int64_t)Np * (P.a_gPhi + 1)];
+            gPhi[2] = (float)ab[(int64_t)Np * (P.a_gPhi + 2)];
+        }
+        const ThermoD t = thermo_d(P, q, Phi);
+        if (write_aux) {
+            double *aw = aux + (int64_t)Np * P.naux * e + n;
+            if (P.a_theta_v >= 0) aw[(int64_t)Np * P.a_theta_v] = t.temp / pow(t.p / P.MSLP, P.kappa);
+            if (P.a_air_T >= 0) aw[(int64_t)Np * P.a_air_T] = t.temp;
+        }
+        if (!MP.ref_sub) {
+            // Without a subtracted reference pressure the momentum flux carries p
+            // relative to an element constant, the pressure at the element's first
+            // node.  A constant flux is annihilated exactly by the weak-form volume
+            // term plus this element's face terms (the LGL summation-by-parts
+            // identity on an affine element), so the offset only removes rounding:
+            // the FP32 flux holds the in-element pressure variation, not p itself.
+            const double *q0 = Q + (int64_t)Np * 5 * e;
+            const double qe[5] = {q0[0], q0[Np], q0[2 * Np], q0[3 * Np], q0[4 * Np]};
+            const double Phe = P.a_Phi >= 0 ? aux[(int64_t)Np * P.naux * e + (int64_t)Np * P.a_Phi] : -0.0;
+            p_e = thermo_d(P, qe, Phe).p;
+        }
+        pf = (float)t.p;
+        pmf = (float)(MP.ref_sub ? t.p - ref_p : t.p - p_e);
+        drho = (float)(MP.ref_src ? q[0] - ref_rho : q[0]);
+        cm = MP.rusanov ? sqrtf(MP.gammaR * (float)t.temp) : 0.0f;
+#pragma unroll
+        for (int s = 0; s < 5; ++s) qf[s] = (float)q[s];
+        const float ri = 1.0f / qf[0];
+        uf[0] = qf[1] * ri;
+        uf[1] = qf[2] * ri;
+        uf[2] = qf[3] * ri;
+        if (DIFF) {
+            const float *gb = gfi + (int64_t)Np * NGF * e + n;
+#pragma unroll
+            for (int s = 0; s < NGF; ++s) gf[s] = gb[Np * s];
+            viscosity_f<SMAG>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
+        }
+        const double Md = ge[3] * A.W[n];
+        MI = (float)(1.0 / Md);
+        const float M = (float)Md;
+#pragma unroll
+        for (int ax = 0; ax < 3; ++ax) {
+            float F[5], F2[5];
+            flux1_row_f(qf, uf, pf, pmf, ax, F);
+            if (DIFF) {
+                flux2_row_f<SMAG>(MP, qf, gf, nu, ax, F2);
+#pragma unroll
+                for (int s = 0; s < 5; ++s) F[s] += F2[s];
+            }
+            const float x = M * (float)ge[ax];
+#pragma unroll
+            for (int s = 0; s < 5; ++s) sF[ax][s][n] = x * F[s];
+        }
+    }
+    __syncthreads();
+    if (!act) return;
+
+    float T[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int m = 0; m < Nq; ++m) {
+            acc += sD[m + Nq * i] * sF[0][s][m + Nq * (j + Nq * k)];
+            acc += sD[m + Nq * j] * sF[1][s][i + Nq * (m + Nq * k)];
+            acc += sD[m + Nq * k] * sF[2][s][i + Nq * (j + Nq * m)];
+        }
+        T[s] = MI * acc;
+    }
+    if (MP.gravity) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) T[1 + d] -= drho * gPhi[d];
+    }
+
+    int fl[3], fnl[3];
+    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
+    for (int c = 0; c < nf; ++c) {
+        const int f = fl[c], d = f >> 1, fn = fnl[c];
+        const float h = (f & 1) ? 0.5f : -0.5f;
+        const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
+        const int32_t ne = A.nbe[6 * e + f];
+        const bool wall = ne < 0;
+        const int fo = f ^ 1;
+        double qp[5];
+        double Php = Phi, refpp = ref_p;
+        if (!wall) {
+#pragma unroll
+            for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
+            Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+            refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 5; ++s) qp[s] = q[s];
+            qp[1 + d] = -q[1 + d];  // reflect the normal momentum
+        }
+        const ThermoD tp = thermo_d(P, qp, Php);
+        float qpf[5], upf[3], jump[5];
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            qpf[s] = (float)qp[s];
+            jump[s] = (float)(q[s] - qp[s]);
+        }
+        const float rip = 1.0f / qpf[0];
+        upf[0] = qpf[1] * rip;
+        upf[1] = qpf[2] * rip;
+        upf[2] = qpf[3] * rip;
+        float Fm[5], Fp[5], flux[5];
+        flux1_row_f(qf, uf, pf, pmf, d, Fm);
+        flux1_row_f(qpf, upf, (float)tp.p, (float)(MP.ref_sub ? tp.p - refpp : tp.p - p_e), d, Fp);
+#pragma unroll
+        for (int s = 0; s < 5; ++s) flux[s] = (Fm[s] + Fp[s]) * h;
+        if (MP.rusanov) {
+            const float umd = d == 0 ? uf[0] : (d == 1 ? uf[1] : uf[2]);
+            const float upd = d == 0 ? upf[0] : (d == 1 ? upf[1] : upf[2]);
+            const float mw = 0.5f * fmaxf(fabsf(umd) + cm, fabsf(upd) + sqrtf(MP.gammaR * (float)tp.temp));
+#pragma unroll
+            for (int s = 0; s < 5; ++s) flux[s] += mw * jump[s];
+        }
+        if (DIFF && !wall) {
+            flux2_row_f<SMAG>(MP, qf, gf, nu, d, Fm);
+#pragma unroll
+            for (int v = 0; v < TR_F2; ++v) flux[1 + v] += (Fm[1 + v] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
+        }
+#pragma unroll
+        for (int s = 0; s < 5; ++s) T[s] -= w * flux[s];
+    }
+
+    double *tb = tend + (int64_t)Np * 5 * e + n;
+    if (MODE == 0) {
+#pragma unroll
+        for (int s = 0; s < 5; ++s) tb[Np * s] = (beta != 0.0) ? alpha * (double)T[s] + beta * tb[Np * s] : alpha * (double)T[s];
+    } else {
+        double *qo = Qout + (int64_t)Np * 5 * e + n;
+        double qn[5];
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            const double dd = (double)T[s] + tb[Np * s];
+            qn[s] = q[s] + bdt * dd;
+            qo[Np * s] = qn[s];
+            tb[Np * s] = dd * rka;
+        }
+        if (qtr_out)
+            for (int c = 0; c < nf; ++c)
+#pragma unroll
+                for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, fl[c], s, TR_Q, fnl[c])] = qn[s];
+    }
+}
+
+}  // namespace cmdg

@@ -196,3 +196,47 @@ def test_update_kernel_order4():
     rate = math.log2(errs[0] / errs[1])
     print(f"  rate {rate:.3f}", flush=True)
     assert abs(rate - 4) < 0.17
+
+
+# ---------------------------------------------------------------- mixed precision
+MIXED_RHS_RTOL = 1e-5   # FP32 fluxes/derivatives vs the FP64 oracle; measured max 2.3e-6 (mass, energy)
+
+
+@pytest.mark.parametrize("name", ["euler_periodic", "viscous_periodic", "les_walls", "les_central"])
+def test_mixed_rhs_error(name):
+    """CMDG_MIXED right-hand side vs the FP64 oracle: relative L2 error per component."""
+    import torch
+    from cmdg.dg import DGModel, to_device, to_host
+    print(f"[mixed] rhs {name}", flush=True)
+    cfg, grid, aux, Q0 = _case(name)
+    ref = O.Oracle(cfg, grid, aux).tendency(Q0)
+    dg = DGModel(cfg, grid, aux=aux, precision=abi.MIXED)
+    Q = to_device(Q0)
+    dQ = torch.full_like(Q, float("nan"))
+    dg(dQ, Q, None, 0.0)
+    got = to_host(dQ)
+    errs = [rel_l2(got[:, s], ref[:, s]) for s in range(5)]
+    print("  rel L2 per component: " + " ".join(f"{r:.2e}" for r in errs), flush=True)
+    for s, r in enumerate(errs):
+        assert r < MIXED_RHS_RTOL, (name, s, r)
+    dg.close()
+
+
+def test_mixed_lsrk_steps_error():
+    """Ten fused mixed-precision LSRK54 steps of the LES case against ten FP64 steps:
+    relative L2 error of the change of each state component."""
+    import torch
+    from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
+    print("[mixed] lsrk steps", flush=True)
+    cfg, grid, aux, Q0 = _case("les_walls")
+    out = {}
+    for prec in (abi.FP64, abi.MIXED):
+        dg = DGModel(cfg, grid, aux=aux, precision=prec)
+        Q = to_device(Q0)
+        solve(Q, LSRK54CarpenterKennedy(dg, Q, dt=5e-3), numberofsteps=10, timeend=math.inf)
+        out[prec] = to_host(Q)
+        dg.close()
+    errs = [rel_l2(out[abi.MIXED][:, s] - Q0[:, s], out[abi.FP64][:, s] - Q0[:, s]) for s in range(5)]
+    print("  rel L2 of the change: " + " ".join(f"{r:.2e}" for r in errs), flush=True)
+    for r in errs:
+        assert r < 1e-5   # measured max 2.9e-7 (energy)
