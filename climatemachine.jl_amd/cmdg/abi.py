@@ -68,19 +68,25 @@ class CmdgDesc(C.Structure):
 EXPORTED = (
     "cmdg_create", "cmdg_destroy", "cmdg_rhs", "cmdg_lsrk54_step",
     "cmdg_lsrk_stage", "cmdg_lsrk_update", "cmdg_aux", "cmdg_gradient_flux",
-    "cmdg_mlp_set_weights", "cmdg_geometry_kind", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
+    "cmdg_gradient_flux_f32", "cmdg_mlp_set_weights", "cmdg_mlp_forward", "cmdg_velocity_gradient",
+    "cmdg_box_filter", "cmdg_closure_samples", "cmdg_mlp_loss_grad", "cmdg_mlp_adam", "cmdg_interpolate_equispaced",
+    "cmdg_geometry_kind", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
     "cmdg_last_error", "cmdg_version",
 )
+MLP_NPARAM = 1441
 KERNEL_KINDS = ("gradient", "tendency", "update", "copy")
 
 _lib = None
 
 
-def load_library(path: str = LIB_PATH):
-    """Load the HIP library once; raise loudly when it is absent."""
+def load_library(path: str = None):
+    """Load the HIP library once; raise loudly when it is absent.  CMDG_LIBRARY in
+    the environment names another build of the same library."""
     global _lib
     if _lib is not None:
         return _lib
+    if path is None:
+        path = os.environ.get("CMDG_LIBRARY", LIB_PATH)
     if not os.path.exists(path):
         raise RuntimeError(
             f"HIP library not built: {path} is missing (run __graft_entry__.build())")
@@ -94,7 +100,18 @@ def load_library(path: str = LIB_PATH):
     lib.cmdg_lsrk_update.argtypes = [vp, vp, C.c_int64, C.c_double, C.c_double, C.c_double, vp]
     lib.cmdg_aux.argtypes = [vp, C.POINTER(vp)]
     lib.cmdg_gradient_flux.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_int32)]
+    lib.cmdg_gradient_flux_f32.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_int32)]
     lib.cmdg_mlp_set_weights.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    lib.cmdg_mlp_forward.argtypes = [vp, vp, vp, C.c_int64]
+    lib.cmdg_velocity_gradient.argtypes = [vp, vp, vp]
+    lib.cmdg_box_filter.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp, vp]
+    lib.cmdg_closure_samples.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_double, C.c_double, vp, C.c_int64,
+                                         C.c_int64, vp]
+    lib.cmdg_mlp_loss_grad.argtypes = [vp, vp, vp, vp, C.c_int64, vp, C.c_int64, C.c_double, vp, vp, C.c_int64, vp]
+    lib.cmdg_mlp_adam.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int64,
+                                  vp]
+    lib.cmdg_interpolate_equispaced.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, C.c_int64, C.c_int64,
+                                                C.c_int64, vp, vp]
     lib.cmdg_geometry_kind.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.cmdg_set_kernel_timing.argtypes = [vp, C.c_int]
     lib.cmdg_kernel_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]

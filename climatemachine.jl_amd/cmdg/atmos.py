@@ -149,7 +149,9 @@ class AtmosConfig:
 
     @property
     def ngradflux(self) -> int:
-        return 10 if self.turbulence in (abi.SMAGORINSKY_LILLY, abi.LEARNED_CLOSURE) else 9
+        if self.turbulence == abi.LEARNED_CLOSURE:
+            return 11      # Smagorinsky-Lilly's 10 + the learned nu_0 (mixed-precision path)
+        return 10 if self.turbulence == abi.SMAGORINSKY_LILLY else 9
 
     def descriptor(self, grid: BrickGrid, aux: np.ndarray, precision: int = abi.FP64):
         """cmdg_desc for this model on ``grid``; the returned tuple keeps the arrays alive."""

@@ -120,6 +120,39 @@ class DGModel:
         self._check(self.lib.cmdg_gradient_flux(self._ctx, C.byref(p), C.byref(n)), "cmdg_gradient_flux")
         return _wrap_device(p.value, (self.nelem, n.value, self.Np), self.device)
 
+    @property
+    def state_gradient_flux_f32(self) -> torch.Tensor:
+        """Device view (nelem, ngradflux, Np) of the mixed-precision FP32 gradient flux."""
+        p = C.c_void_p()
+        n = C.c_int32()
+        self._check(self.lib.cmdg_gradient_flux_f32(self._ctx, C.byref(p), C.byref(n)), "cmdg_gradient_flux_f32")
+        return _wrap_device(p.value, (self.nelem, n.value, self.Np), self.device, "<f4")
+
+    def set_closure(self, mlp) -> None:
+        """Load a learned-closure network (an ``closure.MLPParams``) into the context."""
+        w1, b1, w2, b2, w3, b3, shift, scale = mlp.export()
+        keep = (w1, b1, w2, b2, w3, b3, shift, scale)
+        ptrs = [C.c_void_p(a.ctypes.data) for a in keep]
+        self._check(self.lib.cmdg_mlp_set_weights(self._ctx, *ptrs), "cmdg_mlp_set_weights")
+        del keep
+
+    def closure_forward(self, x_raw: torch.Tensor) -> torch.Tensor:
+        """The context's network on raw input rows (n, 10) float32 CUDA -> (n,)."""
+        if x_raw.dtype != torch.float32 or not x_raw.is_contiguous() or not x_raw.is_cuda:
+            raise CmdgError("inputs must be a contiguous float32 CUDA tensor")
+        y = torch.empty(x_raw.shape[0], dtype=torch.float32, device=x_raw.device)
+        self._check(self.lib.cmdg_mlp_forward(self._ctx, C.c_void_p(x_raw.data_ptr()), C.c_void_p(y.data_ptr()),
+                                              int(x_raw.shape[0])), "cmdg_mlp_forward")
+        return y
+
+    def velocity_gradient(self, Q: torch.Tensor, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """du_c/dx_d of Q as the gradient pass forms it: (nelem, 9, Np), entry 3c + d."""
+        if out is None:
+            out = torch.empty((self.nelem, 9, self.Np), dtype=torch.float64, device=Q.device)
+        self._check(self.lib.cmdg_velocity_gradient(self._ctx, C.c_void_p(_ptr(Q)), C.c_void_p(_ptr(out))),
+                    "cmdg_velocity_gradient")
+        return out
+
     def init_ode_state(self, Q0: np.ndarray) -> torch.Tensor:
         """Upload an initial state built on the host (init_ode_state with init_on_cpu)."""
         return to_device(Q0, self.device)
@@ -172,12 +205,12 @@ class DGModel:
             pass
 
 
-def _wrap_device(ptr: int, shape, device) -> torch.Tensor:
+def _wrap_device(ptr: int, shape, device, typestr: str = "<f8") -> torch.Tensor:
     """A torch view of library-owned device memory (valid while the context lives)."""
     n = int(np.prod(shape))
 
     class _Iface:
-        __cuda_array_interface__ = {"shape": (n,), "typestr": "<f8", "data": (ptr, False), "version": 2,
+        __cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False), "version": 2,
                                     "strides": None}
     with torch.cuda.device(device):
         return torch.as_tensor(_Iface(), device=device).view(*shape)

@@ -37,6 +37,7 @@
 #include "cmdg_general.h"
 #include "cmdg_affine.h"
 #include "cmdg_mixed.h"
+#include "cmdg_learn.h"
 
 namespace {
 using namespace cmdg;
@@ -88,6 +89,9 @@ struct cmdg_ctx {
     bool mixed = false;
     MixParams MP{};
     float *d_gf32 = nullptr, *d_f2tr32 = nullptr;
+    // learned closure (mixed precision only)
+    bool learned = false, mlp_ready = false;
+    MlpDev *d_mlp = nullptr;
     std::string err;
     // optional per-kernel event timing (cmdg_set_kernel_timing)
     bool timing = false;
@@ -152,12 +156,15 @@ template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, const d
     dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_GRADIENT);
     if (c->mixed) {
-        if (c->smag)
-            hipLaunchKernelGGL((k_gradient_mix<Nq, true>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem, Q,
-                               qtr, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux);
+        if (c->learned)
+            hipLaunchKernelGGL((k_gradient_mix<Nq, true, true>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem,
+                               Q, qtr, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
+        else if (c->smag)
+            hipLaunchKernelGGL((k_gradient_mix<Nq, true, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem,
+                               Q, qtr, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
         else
-            hipLaunchKernelGGL((k_gradient_mix<Nq, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem, Q,
-                               qtr, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux);
+            hipLaunchKernelGGL((k_gradient_mix<Nq, false, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA,
+                               c->nelem, Q, qtr, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
     } else if (c->affine) {
         if (c->smag)
             hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q, qtr,
@@ -178,12 +185,14 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_TENDENCY);
+#define CMDG_TEND_MIX(DIFF, SMAG, LEARN, MODE)                                                                   \
+    hipLaunchKernelGGL((k_tendency_mix<Nq, DIFF, SMAG, LEARN, MODE>), grid, block, 0, c->stream, c->P, c->MP, c->AA, \
+                       c->nelem, Q, qtr, c->d_gf32, c->d_f2tr32, c->d_aux, tend, alpha, beta, Qout, qtr_out, rka,   \
+                       bdt, write_aux)
 #define CMDG_TEND(DIFF, SMAG, MODE)                                                                               \
     do {                                                                                                          \
         if (c->mixed)                                                                                             \
-            hipLaunchKernelGGL((k_tendency_mix<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->MP,    \
-                               c->AA, c->nelem, Q, qtr, c->d_gf32, c->d_f2tr32, c->d_aux, tend, alpha, beta,      \
-                               Qout, qtr_out, rka, bdt, write_aux);                                               \
+            CMDG_TEND_MIX(DIFF, SMAG, false, MODE);                                                               \
         else if (c->affine)                                                                                       \
             hipLaunchKernelGGL((k_tendency_aff<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->AA,    \
                                c->nelem, Q, qtr, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, qtr_out,  \
@@ -192,7 +201,10 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double
             hipLaunchKernelGGL((k_tendency<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->A, Q,      \
                                c->d_gf, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux);                 \
     } while (0)
-    if (mode == 0) {
+    if (c->learned) {
+        if (mode == 0) CMDG_TEND_MIX(true, true, true, 0);
+        else CMDG_TEND_MIX(true, true, true, 1);
+    } else if (mode == 0) {
         if (!c->diffusive) CMDG_TEND(false, false, 0);
         else if (c->smag) CMDG_TEND(true, true, 0);
         else CMDG_TEND(true, false, 0);
@@ -202,6 +214,7 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double
         else CMDG_TEND(true, false, 1);
     }
 #undef CMDG_TEND
+#undef CMDG_TEND_MIX
     return check_hip(c, hipGetLastError(), "tendency launch");
 }
 
@@ -234,6 +247,7 @@ int launch_tendency(cmdg_ctx *c, const double *Q, const double *qtr, double *ten
 // dQ/dt into tend (mode 0) with the auxiliary refresh of DGModel.jl:110-116
 int launch_rhs(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta) {
     int rc;
+    if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
     const double *qtr = nullptr;
     if (c->affine) {
         if ((rc = launch_trace(c, Q, c->d_qtr[0], nullptr))) return rc;
@@ -355,7 +369,8 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     if (!d->vgeo || !d->sgeo || !d->vmap_minus || !d->vmap_plus || !d->elemtobndy || !d->D)
         return fail(c, CMDG_EINVAL, "missing geometry or connectivity");
     if (d->precision != CMDG_FP64 && d->precision != CMDG_MIXED) return fail(c, CMDG_EINVAL, "unknown precision");
-    if (d->turbulence == CMDG_LEARNED_CLOSURE) return fail(c, CMDG_EUNSUPPORTED, "learned closure not built");
+    if (d->turbulence == CMDG_LEARNED_CLOSURE && d->precision != CMDG_MIXED)
+        return fail(c, CMDG_EUNSUPPORTED, "the learned closure runs in the mixed-precision path");
     if (d->numerical_flux != CMDG_RUSANOV && d->numerical_flux != CMDG_CENTRAL)
         return fail(c, CMDG_EUNSUPPORTED, "numerical flux");
     auto in_aux = [&](int32_t off, int width) { return off < 0 || off + width <= d->naux; };
@@ -367,8 +382,9 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
         return fail(c, CMDG_EINVAL, "FlatOrientation needs Phi and grad Phi in aux");
     if (d->ref_state == CMDG_HYDROSTATIC_STATE && (d->aux_ref_rho < 0 || d->aux_ref_p < 0))
         return fail(c, CMDG_EINVAL, "HydrostaticState needs ref rho and p in aux");
-    if (d->turbulence == CMDG_SMAGORINSKY_LILLY && (d->aux_Delta < 0 || d->aux_theta_v < 0))
-        return fail(c, CMDG_EINVAL, "Smagorinsky-Lilly needs Delta and theta_v in aux");
+    if ((d->turbulence == CMDG_SMAGORINSKY_LILLY || d->turbulence == CMDG_LEARNED_CLOSURE) &&
+        (d->aux_Delta < 0 || d->aux_theta_v < 0))
+        return fail(c, CMDG_EINVAL, "Smagorinsky-Lilly and the learned closure need Delta and theta_v in aux");
 
     c->N = d->N;
     c->Nq = d->N + 1;
@@ -376,9 +392,10 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     c->Nfp = c->Nq * c->Nq;
     c->nelem = d->nelem;
     if ((double)c->nelem * c->Np >= 2147483647.0) return fail(c, CMDG_EUNSUPPORTED, "mesh too large for int32 node ids");
-    c->smag = d->turbulence == CMDG_SMAGORINSKY_LILLY;
+    c->learned = d->turbulence == CMDG_LEARNED_CLOSURE;
+    c->smag = d->turbulence == CMDG_SMAGORINSKY_LILLY || c->learned;
     c->diffusive = c->smag || d->viscosity != 0.0;
-    c->ngf = c->smag ? 10 : 9;
+    c->ngf = c->learned ? 11 : (c->smag ? 10 : 9);
     c->stream = (hipStream_t)stream;
 
     KParams &P = c->P;
@@ -505,6 +522,7 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
         M.rusanov = P.rusanov;
         if ((rc = upload<float>(c, &c->d_gf32, nullptr, (size_t)Np * c->ngf * E))) return rc;
         if ((rc = upload<float>(c, &c->d_f2tr32, nullptr, (size_t)E * 6 * TR_F2 * Nfp))) return rc;
+        if (c->learned && (rc = upload<MlpDev>(c, &c->d_mlp, nullptr, 1))) return rc;
     }
     return CMDG_OK;
 }
@@ -533,6 +551,7 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_str);
     (void)hipFree(c->d_gf32);
     (void)hipFree(c->d_f2tr32);
+    (void)hipFree(c->d_mlp);
     delete c;
     return CMDG_OK;
 }
@@ -574,6 +593,7 @@ int cmdg_lsrk54_step(cmdg_ctx *c, double *Q, double *dQ, double t, double dt) {
     // result lands in the caller's Q.  On affine bricks the state's face trace is
     // ping-ponged alongside: written from Q by the trace kernel, then by each
     // stage's epilogue for the next stage.
+    if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
     double *C = Q, *S = c->d_Qs;
     const int64_t n = (int64_t)c->Np * 5 * c->nelem;
     int rc;
@@ -617,9 +637,137 @@ int cmdg_gradient_flux(cmdg_ctx *c, double **gf, int32_t *ngf) {
     return CMDG_OK;
 }
 
-int cmdg_mlp_set_weights(cmdg_ctx *c, const uint16_t *, const float *, const uint16_t *, const float *,
-                         const uint16_t *, const float *, const float *, const float *) {
-    return fail(c, CMDG_EUNSUPPORTED, "learned closure not built");
+int cmdg_gradient_flux_f32(cmdg_ctx *c, float **gf, int32_t *ngf) {
+    if (!c || !gf) return CMDG_EINVAL;
+    if (!c->mixed) return fail(c, CMDG_EUNSUPPORTED, "the FP64 path stores its gradient flux in FP64");
+    *gf = c->d_gf32;
+    if (ngf) *ngf = c->ngf;
+    return CMDG_OK;
+}
+
+int cmdg_mlp_set_weights(cmdg_ctx *c, const uint16_t *w1, const float *b1, const uint16_t *w2, const float *b2,
+                         const uint16_t *w3, const float *b3, const float *shift, const float *scale) {
+    if (!c) return CMDG_EINVAL;
+    if (!c->learned) return fail(c, CMDG_EINVAL, "the context was not created with the learned closure");
+    if (!w1 || !b1 || !w2 || !b2 || !w3 || !b3 || !shift || !scale) return fail(c, CMDG_EINVAL, "null pointer");
+    MlpDev h{};
+    for (int o = 0; o < MLP_H; ++o) {
+        for (int k = 0; k < MLP_IN; ++k) h.w1[o * MLP_K1 + k] = w1[o * MLP_IN + k];
+        for (int k = 0; k < MLP_H; ++k) h.w2[o * MLP_H + k] = w2[o * MLP_H + k];
+        h.b1[o] = b1[o];
+        h.b2[o] = b2[o];
+        const uint32_t bits = (uint32_t)w3[o] << 16;
+        std::memcpy(&h.w3[o], &bits, 4);
+    }
+    h.b3 = b3[0];
+    for (int k = 0; k < MLP_IN; ++k) {
+        h.shift[k] = shift[k];
+        h.scale[k] = scale[k];
+    }
+    int rc = check_hip(c, hipMemcpyAsync(c->d_mlp, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+    if (rc) return rc;
+    rc = check_hip(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    if (!rc) c->mlp_ready = true;
+    return rc;
+}
+
+int cmdg_mlp_forward(cmdg_ctx *c, const float *x_raw, float *y, int64_t n) {
+    if (!c) return CMDG_EINVAL;
+    if (!c->learned || !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
+    if (n < 0 || (n > 0 && (!x_raw || !y))) return fail(c, CMDG_EINVAL, "bad arguments");
+    if (n == 0) return CMDG_OK;
+    const int64_t nb = (n + 127) / 128;
+    hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)nb), dim3(128), 0, c->stream, c->d_mlp, x_raw, y, n);
+    return check_hip(c, hipGetLastError(), "mlp forward launch");
+}
+
+int cmdg_velocity_gradient(cmdg_ctx *c, const double *Q, double *grad_u) {
+    if (!c) return CMDG_EINVAL;
+    if (!Q || !grad_u) return fail(c, CMDG_EINVAL, "null pointer");
+    if (!c->affine) return fail(c, CMDG_EUNSUPPORTED, "the velocity gradient is built for affine brick meshes");
+    int rc = launch_trace(c, Q, c->d_qtr[0], nullptr);
+    if (rc) return rc;
+    dim3 grid((unsigned)xcd_grid(c->nelem));
+    if (c->Nq == 5)
+        hipLaunchKernelGGL(k_velgrad_aff<5>, grid, dim3(threads_for<5>()), 0, c->stream, c->AA, c->nelem, Q, c->d_qtr[0], grad_u);
+    else
+        hipLaunchKernelGGL(k_velgrad_aff<4>, grid, dim3(threads_for<4>()), 0, c->stream, c->AA, c->nelem, Q, c->d_qtr[0], grad_u);
+    return check_hip(c, hipGetLastError(), "velocity gradient launch");
+}
+
+int cmdg_box_filter(const double *Qf, const int32_t *children, int64_t ncoarse, int32_t N, const double *proj,
+                    double *moments, void *stream) {
+    if ((N != 3 && N != 4) || ncoarse < 0 || !proj) return CMDG_EINVAL;
+    if (ncoarse == 0) return CMDG_OK;
+    if (!Qf || !children || !moments) return CMDG_EINVAL;
+    const int Nq = N + 1;
+    FilterProj PR{};
+    for (int h = 0; h < 2; ++h)
+        for (int i = 0; i < Nq; ++i)
+            for (int a = 0; a < Nq; ++a) PR.p[h][i][a] = proj[(h * Nq + i) * Nq + a];
+    if (Nq == 5)
+        hipLaunchKernelGGL(k_box_filter<5>, dim3((unsigned)ncoarse), dim3(threads_for<5>()), 0, (hipStream_t)stream, Qf,
+                           children, ncoarse, PR, moments);
+    else
+        hipLaunchKernelGGL(k_box_filter<4>, dim3((unsigned)ncoarse), dim3(threads_for<4>()), 0, (hipStream_t)stream, Qf,
+                           children, ncoarse, PR, moments);
+    return hipGetLastError() == hipSuccess ? CMDG_OK : CMDG_EHIP;
+}
+
+int cmdg_closure_samples(const double *moments, const double *grad_u, int64_t nelem, int32_t N, double Delta,
+                         double nu_mol, float *samples, int64_t ld, int64_t offset, void *stream) {
+    if ((N != 3 && N != 4) || nelem < 0 || offset < 0) return CMDG_EINVAL;
+    const int Np = (N + 1) * (N + 1) * (N + 1);
+    const int64_t n = nelem * Np;
+    if (offset + n > ld) return CMDG_EINVAL;
+    if (n == 0) return CMDG_OK;
+    if (!moments || !grad_u || !samples) return CMDG_EINVAL;
+    hipLaunchKernelGGL(k_closure_samples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, moments,
+                       grad_u, nelem, Np, Delta, nu_mol, samples, ld, offset);
+    return hipGetLastError() == hipSuccess ? CMDG_OK : CMDG_EHIP;
+}
+
+int cmdg_mlp_loss_grad(const float *params, const float *shift, const float *scale, const float *samples, int64_t ld,
+                       const int64_t *idx, int64_t batch, double inv_norm, float *grad, float *work,
+                       int64_t work_floats, void *stream) {
+    if (batch <= 0 || !params || !shift || !scale || !samples || !grad || !work) return CMDG_EINVAL;
+    if (!idx && batch > ld) return CMDG_EINVAL;
+    const int64_t nb = (batch + TRAIN_TB - 1) / TRAIN_TB;
+    if (nb * GRAD_W > work_floats) return CMDG_EINVAL;
+    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)nb), dim3(TRAIN_TB), 0, (hipStream_t)stream, params, shift, scale,
+                       samples, ld, idx, batch, (float)inv_norm, work);
+    if (hipGetLastError() != hipSuccess) return CMDG_EHIP;
+    hipLaunchKernelGGL(k_grad_reduce, dim3((GRAD_W + 255) / 256), dim3(256), 0, (hipStream_t)stream, work, nb, grad);
+    return hipGetLastError() == hipSuccess ? CMDG_OK : CMDG_EHIP;
+}
+
+int cmdg_mlp_adam(float *params, const float *grad, float *m, float *v, int64_t n, float lr, float beta1, float beta2,
+                  float eps, int64_t step, void *stream) {
+    if (n <= 0 || step < 1 || !params || !grad || !m || !v) return CMDG_EINVAL;
+    const float c1 = (float)(1.0 - std::pow((double)beta1, (double)step));
+    const float c2 = (float)(1.0 - std::pow((double)beta2, (double)step));
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, params, grad, m, v, n,
+                       lr, beta1, beta2, eps, c1, c2);
+    return hipGetLastError() == hipSuccess ? CMDG_OK : CMDG_EHIP;
+}
+
+int cmdg_interpolate_equispaced(const double *Q, const int32_t *eijk, int64_t nelem, int32_t N, int32_t p,
+                                const double *L, int64_t nx, int64_t ny, int64_t nz, double *u, void *stream) {
+    if ((N != 3 && N != 4) || p < 1 || p > 8 || nelem < 0 || !L) return CMDG_EINVAL;
+    if (nelem == 0) return CMDG_OK;
+    if (!Q || !eijk || !u || nx <= 0 || ny <= 0 || nz <= 0) return CMDG_EINVAL;
+    const int Nq = N + 1;
+    InterpMat IM{};
+    for (int m = 0; m < p; ++m)
+        for (int i = 0; i < Nq; ++i) IM.L[m][i] = L[m * Nq + i];
+    const unsigned nt = (unsigned)(((p * p * p) + 63) / 64 * 64);
+    if (Nq == 5)
+        hipLaunchKernelGGL(k_interp_equispaced<5>, dim3((unsigned)nelem), dim3(nt < 128 ? 128 : nt), 0,
+                           (hipStream_t)stream, Q, eijk, nelem, p, IM, nx, ny, nz, u);
+    else
+        hipLaunchKernelGGL(k_interp_equispaced<4>, dim3((unsigned)nelem), dim3(nt < 64 ? 64 : nt), 0,
+                           (hipStream_t)stream, Q, eijk, nelem, p, IM, nx, ny, nz, u);
+    return hipGetLastError() == hipSuccess ? CMDG_OK : CMDG_EHIP;
 }
 
 int cmdg_geometry_kind(cmdg_ctx *c, int32_t *kind) {

@@ -12,6 +12,7 @@
 // formed in FP64.  Metric terms are computed once in FP64 and rounded on use.
 #pragma once
 #include "cmdg_affine.h"
+#include "cmdg_closure.h"
 
 namespace cmdg {
 
@@ -31,20 +32,26 @@ __device__ __forceinline__ void flux1_row_f(const float q[5], const float u[3], 
     F[4] = ud * (q[4] + p);
 }
 
-template <bool SMAG>
+__device__ __forceinline__ float strain_norm_f(const float *gf) {
+#pragma clang fp contract(fast)
+    const float *S = gf + 3;
+    const float norm2 = S[0] * S[0] + S[3] * S[3] + S[5] * S[5] + 2.0f * (S[1] * S[1] + S[2] * S[2] + S[4] * S[4]);
+    return sqrtf(2.0f * norm2);
+}
+
+// Smagorinsky-Lilly viscosity; LEARN takes nu_0 from gf[10] (the learned closure).
+template <bool SMAG, bool LEARN = false>
 __device__ __forceinline__ void viscosity_f(const MixParams &P, float rho, const float *gf, const float gPhi[3],
                                             float Delta, float nu[3]) {
 #pragma clang fp contract(fast)
     if (SMAG) {
-        const float *S = gf + 3;
-        const float norm2 = S[0] * S[0] + S[3] * S[3] + S[5] * S[5] + 2.0f * (S[1] * S[1] + S[2] * S[2] + S[4] * S[4]);
-        const float normS = sqrtf(2.0f * norm2);
+        const float normS = strain_norm_f(gf);
         const float ig = 1.0f / P.grav;
         const float k[3] = {gPhi[0] * ig, gPhi[1] * ig, gPhi[2] * ig};
         const float Ri = gf[9] / (normS * normS + (nextafterf(normS, INFINITY) - normS));
         const float fb2 = sqrtf(fminf(fmaxf(1.0f - Ri * P.inv_Pr, 0.0f), 1.0f));
         const float CD = P.C_smag * Delta;
-        const float nu0 = normS * (CD * CD) + 1e-5f;
+        const float nu0 = LEARN ? gf[10] : normS * (CD * CD) + 1e-5f;
         const float dotk = nu0 * (k[0] + k[1] + k[2]);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -125,16 +132,18 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 }
 
 // ------------------------------------------------------------------ gradient (mixed)
-template <int Nq, bool SMAG>
+template <int Nq, bool SMAG, bool LEARN>
 __global__ void __launch_bounds__(threads_for<Nq>())
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
                const double *__restrict__ qtr, double *__restrict__ aux, float *__restrict__ gfo,
-               float *__restrict__ f2tr, int write_aux) {
+               float *__restrict__ f2tr, int write_aux, const MlpDev *__restrict__ mlp) {
 #pragma clang fp contract(fast)
+    static_assert(!LEARN || SMAG, "the learned closure keeps the Smagorinsky-Lilly structure");
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
-    constexpr int NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
+    constexpr int NG = SMAG ? 5 : 4, NGF = LEARN ? 11 : (SMAG ? 10 : 9);
     __shared__ float sD[Nq * Nq];
     __shared__ float sG[NG][Np];
+    __shared__ uint16_t sX[LEARN ? (threads_for<Nq>() / 64) * MLP_LDS_PER_WAVE : 1];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     const int n = threadIdx.x;
@@ -168,10 +177,13 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         for (int s = 0; s < NG; ++s) sG[s][n] = G[s];
     }
     __syncthreads();
-    if (!act) return;
+    if (!LEARN && !act) return;
 
     const float xi0 = (float)ge[0], xi1 = (float)ge[1], xi2 = (float)ge[2];
-    float g[3][5];
+    float g[3][5] = {};
+    int fl[3], fnl[3], nf = 0;
+    float gf[11];
+    if (act) {
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
         float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
@@ -189,8 +201,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     }
     // lifted central jumps, accumulated into the gradient: M^-1 Mf n (G* - G-)
     const float MI = (float)(1.0 / (ge[3] * A.W[n]));
-    int fl[3], fnl[3];
-    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
+    nf = node_faces<Nq>(i, j, k, fl, fnl);
     for (int c = 0; c < nf; ++c) {
         const int f = fl[c], d = f >> 1, fn = fnl[c];
         const float nd = (f & 1) ? 1.0f : -1.0f;
@@ -216,8 +227,20 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 #pragma unroll
         for (int s = 0; s < 5; ++s) g[d][s] += (w * nd) * jump[s];
     }
-    float gf[10];
     grad_flux_f<SMAG>(g, gPhi, thv, gf);
+    }
+    if (LEARN) {
+        // the network runs on the assembled gradient (volume + lifted face jumps):
+        // it is nonlinear, so it cannot be split over the two contributions
+        float x[MLP_IN];
+        mlp_inputs(mlp, g, (float)ge[10], x);
+        if (!act)
+#pragma unroll
+            for (int c = 0; c < MLP_IN; ++c) x[c] = 0.0f;
+        const float y = mlp_wave(mlp, x, sX);
+        gf[10] = mlp_nu0(y, (float)ge[10], strain_norm_f(gf));
+        if (!act) return;
+    }
     float *go = gfo + (int64_t)Np * NGF * e + n;
 #pragma unroll
     for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
@@ -226,7 +249,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         float nu[3], qf[5];
 #pragma unroll
         for (int s = 0; s < 5; ++s) qf[s] = (float)q[s];
-        viscosity_f<SMAG>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
+        viscosity_f<SMAG, LEARN>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
         for (int c = 0; c < nf; ++c) {
             float F2[5];
             flux2_row_f<SMAG>(MP, qf, gf, nu, fl[c] >> 1, F2);
@@ -237,15 +260,15 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 }
 
 // ------------------------------------------------------------------ tendency (mixed)
-template <int Nq, bool DIFF, bool SMAG, int MODE>
+template <int Nq, bool DIFF, bool SMAG, bool LEARN, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>())
 k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
                const double *__restrict__ qtr, const float *__restrict__ gfi, const float *__restrict__ f2tr,
                double *__restrict__ aux, double *__restrict__ tend, double alpha, double beta,
                double *__restrict__ Qout, double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
-#pragma clang fp contract(fast)
+
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
-    constexpr int NGF = SMAG ? 10 : 9;
+    constexpr int NGF = LEARN ? 11 : (SMAG ? 10 : 9);
     __shared__ float sD[Nq * Nq];
     __shared__ float sF[3][5][Np];
     const int64_t e = xcd_element(blockIdx.x, nelem);
@@ -257,7 +280,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     const double *ge = A.geo + (int64_t)AFF_GEO * e;
 
     double q[5], Phi = -0.0, ref_p = 0.0, p_e = 0.0;
-    float qf[5], uf[3], gf[10], nu[3], gPhi[3] = {0.0f, 0.0f, 0.0f}, pf = 0.0f, pmf = 0.0f, cm = 0.0f, drho = 0.0f;
+    float qf[5], uf[3], gf[11], nu[3], gPhi[3] = {0.0f, 0.0f, 0.0f}, pf = 0.0f, pmf = 0.0f, cm = 0.0f, drho = 0.0f;
     float MI = 0.0f;
     if (act) {
         const double *qb = Q + (int64_t)Np * 5 * e + n;
@@ -304,7 +327,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
             const float *gb = gfi + (int64_t)Np * NGF * e + n;
 #pragma unroll
             for (int s = 0; s < NGF; ++s) gf[s] = gb[Np * s];
-            viscosity_f<SMAG>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
+            viscosity_f<SMAG, LEARN>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
         }
         const double Md = ge[3] * A.W[n];
         MI = (float)(1.0 / Md);

@@ -137,12 +137,76 @@ int cmdg_aux(cmdg_ctx *ctx, double **aux_device);
  * FP64; ngradflux is 9 (constant viscosity) or 10 (Smagorinsky). */
 int cmdg_gradient_flux(cmdg_ctx *ctx, double **gradflux_device, int32_t *ngradflux);
 
-/* Set the learned-closure MLP: 10 inputs -> 32 -> 32 -> 1 (SiLU), BF16 weights
- * (row-major [out][in]) and FP32 biases plus FP32 input normalisation. */
+/* Device pointer to the FP32 gradient flux of the last mixed-precision RHS
+ * (Np, ngradflux, nelem): 9 (constant viscosity), 10 (Smagorinsky-Lilly) or 11
+ * (learned closure: entry 10 holds the learned nu_0). */
+int cmdg_gradient_flux_f32(cmdg_ctx *ctx, float **gradflux_device, int32_t *ngradflux);
+
+/* ---------------------------------------------------------------------------
+ * Learned closure (no reference counterpart; it replaces nu_0 of
+ * SmagorinskyLilly, src/Common/TurbulenceClosures/TurbulenceClosures.jl:478-491).
+ * A per-node MLP 10 -> 32 -> 32 -> 1 with SiLU hidden units.  Raw inputs: the
+ * velocity gradient du_c/dx_d at index 3c + d and the filter width Delta at 9;
+ * the network sees x = (raw - shift) * scale; its output y gives
+ *     nu_0 = exp(min(y, 8)) * Delta^2 * |S| + 1e-5,   |S| = sqrt(2 S:S).
+ * Weights are BF16 (row-major [out][in]: w1 32x10, w2 32x32, w3 1x32), biases FP32
+ * (b1 32, b2 32, b3 1), shift/scale 10 FP32 each; all host pointers.  Evaluated
+ * with BF16 MFMA / FP32 accumulation inside the mixed-precision gradient kernel. */
 int cmdg_mlp_set_weights(cmdg_ctx *ctx, const uint16_t *w1_bf16, const float *b1,
                          const uint16_t *w2_bf16, const float *b2,
                          const uint16_t *w3_bf16, const float *b3,
                          const float *in_shift, const float *in_scale);
+
+/* The network alone on n raw input rows (device, row-major n x 10) -> y (device, n). */
+int cmdg_mlp_forward(cmdg_ctx *ctx, const float *x_raw, float *y, int64_t n);
+
+/* Velocity gradient du_c/dx_d of Q at (node, 3c + d, element), FP64 device
+ * (Np, 9, nelem), formed as the gradient pass forms it (volume derivative plus
+ * lifted central jump, wall rule of bc_momentum.jl:24-43).  Affine bricks. */
+int cmdg_velocity_gradient(cmdg_ctx *ctx, const double *Q, double *grad_u);
+
+/* 2:1 element-agglomeration filter (no reference counterpart): coarse element c is
+ * the union of the 8 fine elements children[8c + hx + 2hy + 4hz] (h = 0: lower
+ * half along that axis).  The coarse nodal values are the L2 projection of the
+ * fine DG field, proj[(h*Nq + i)*Nq + a] the 1-D projection from fine node a of half
+ * h to coarse node i (host pointer).  Filtered: rho, rho u (3), rho e and
+ * rho u_a u_b (11,12,13,22,23,33) -> moments (Np, 11, ncoarse).  Device arrays. */
+int cmdg_box_filter(const double *Q_fine, const int32_t *children, int64_t ncoarse, int32_t N,
+                    const double *proj, double *moments, void *stream);
+
+/* Training samples of the subgrid stress, one per coarse node, SoA with leading
+ * dimension ld starting at column `offset`: rows 0-8 raw velocity gradient, row 9
+ * Delta, rows 10-12 P, Q, R such that the squared deviatoric-stress error of the
+ * closure is exp(2y) P + 2 exp(y) Q + R (the stress the model must supply is the
+ * subgrid stress plus the molecular stress 2 nu_mol S^d). */
+int cmdg_closure_samples(const double *moments, const double *grad_u, int64_t nelem, int32_t N,
+                         double Delta, double nu_mol, float *samples, int64_t ld, int64_t offset,
+                         void *stream);
+
+/* MLP training, FP32 master parameters (CMDG_MLP_NPARAM, flat: W1[32][10], b1[32],
+ * W2[32][32], b2[32], W3[32], b3).  Loss and gradient over a batch of samples
+ * (rows idx[0..batch) of the sample table, or its first `batch` columns when idx is
+ * NULL): grad[0..NPARAM) and grad[NPARAM] = loss = sum (exp(2y)P + 2exp(y)Q + R) *
+ * inv_norm.  work: device scratch of at least ceil(batch/64) * (NPARAM+1) floats. */
+#define CMDG_MLP_NPARAM 1441
+int cmdg_mlp_loss_grad(const float *params, const float *shift, const float *scale,
+                       const float *samples, int64_t ld, const int64_t *idx, int64_t batch,
+                       double inv_norm, float *grad, float *work, int64_t work_floats,
+                       void *stream);
+
+/* One Adam step (bias-corrected, step >= 1) on n parameters. */
+int cmdg_mlp_adam(float *params, const float *grad, float *m, float *v, int64_t n, float lr,
+                  float beta1, float beta2, float eps, int64_t step, void *stream);
+
+/* Velocity u = (rho u)/rho interpolated to p^3 equispaced points per element
+ * (cell centres xi_m = -1 + (2m+1)/p; L[m*Nq + i] = l_i(xi_m), host) and written to
+ * u[c][Z][Y][X] (c = 0..2; X = ix*p + mx, ...; dims nz, ny, nx points) from the
+ * element's brick index eijk[3e..3e+2] = (ix, iy, iz): the uniform-grid input of
+ * power_spectrum_3d (src/Common/Spectra/power_spectrum_les.jl:13-75), as
+ * InterpolationBrick (src/Numerics/Mesh/Interpolation.jl:66-571) supplies it. */
+int cmdg_interpolate_equispaced(const double *Q, const int32_t *eijk, int64_t nelem, int32_t N,
+                                int32_t p, const double *L, int64_t nx, int64_t ny, int64_t nz,
+                                double *u, void *stream);
 
 /* Geometry representation the context chose (diagnostic; no reference counterpart):
  * 1 = axis-aligned affine bricks (per-element metrics, detected bitwise at create),

@@ -15,6 +15,12 @@
 // free-stream preservation, conservation, and closed forms for the gradient
 // and Smagorinsky-Lilly closure on linear velocity fields.  Smagorinsky-Lilly
 // has no reference known-answer value; the closed-form test is its only pin.
+// The learned closure (no reference counterpart) replaces Smagorinsky-Lilly's
+// nu_0 by exp(y) Delta^2 |S| + 1e-5, y the output of a 10-32-32-1 SiLU network on
+// the assembled velocity gradient; its inference here emulates the BF16 rounding
+// of the GPU (inputs, weights, first hidden layer) with exact arithmetic otherwise.
+// Pinned by: a constant-output network reproduces Smagorinsky-Lilly, and the
+// network against an independent numpy evaluation (tests/test_closure_oracle.py).
 #include "../include/cmdg.h"
 
 #include <algorithm>
@@ -46,10 +52,40 @@ template <class T> struct Ctx {
     std::vector<T> vgeo, sgeo, D;
     std::vector<int64_t> vmm, vmp, bndy;
     std::vector<T> gradflux;  // (Np, ngf, nelem)
-    int ngrad, ngf;
-    bool diffusive;
+    int ngrad, ngf, ngf_lin;  // entries [0, ngf_lin) are linear in the gradient
+    bool diffusive, smag, learned;
     T R_d, cp_d, cv_d, T_0, grav, MSLP, visc, C_smag, inv_Pr;
+    // learned closure: BF16-valued weights held as double
+    double w1[320], b1[32], w2[1024], b2[32], w3[32], b3, shift[10], scale[10];
 };
+
+// Round to the nearest BF16 value (ties to even), as the GPU's conversion does.
+inline double bf16_round(double v) {
+    float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    std::memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+// Network output for raw inputs (9 velocity-gradient entries du_c/dx_d at 3c+d, Delta).
+template <class T> double mlp_output(const Ctx<T> &c, const double raw[10]) {
+    double x[10], h1[32], y = c.b3;
+    for (int k = 0; k < 10; ++k) x[k] = bf16_round((double)(float)(((float)raw[k] - (float)c.shift[k]) * (float)c.scale[k]));
+    for (int o = 0; o < 32; ++o) {
+        double a = c.b1[o];
+        for (int k = 0; k < 10; ++k) a += c.w1[10 * o + k] * x[k];
+        h1[o] = bf16_round(a / (1.0 + std::exp(-a)));
+    }
+    for (int o = 0; o < 32; ++o) {
+        double a = c.b2[o];
+        for (int k = 0; k < 32; ++k) a += c.w2[32 * o + k] * h1[k];
+        y += c.w3[o] * (a / (1.0 + std::exp(-a)));
+    }
+    return y;
+}
 
 // vgeo ids (src/Numerics/Mesh/Grids.jl:76-92): xi_a x_d at a + 3 d
 enum { VM = 9, VMI = 10, VX1 = 12 };
@@ -139,7 +175,7 @@ template <class T> inline Turb<T> turbulence(const Ctx<T> &c, const T *q, const 
     Turb<T> tb;
     const T *S = gf + 3;
     T Sm[3][3] = {{S[0], S[1], S[2]}, {S[1], S[3], S[4]}, {S[2], S[4], S[5]}};
-    if (c.d.turbulence == CMDG_SMAGORINSKY_LILLY) {
+    if (c.smag) {
         T norm2 = ((((S[0] * S[0] + T(2) * (S[1] * S[1])) + T(2) * (S[2] * S[2])) + S[3] * S[3]) +
                    T(2) * (S[4] * S[4])) + S[5] * S[5];
         T normS = std::sqrt(T(2) * norm2);
@@ -147,7 +183,7 @@ template <class T> inline Turb<T> turbulence(const Ctx<T> &c, const T *q, const 
         T Ri = gf[9] / (normS * normS + julia_eps(normS));
         T fb2 = std::sqrt(std::min(std::max(T(1) - Ri * c.inv_Pr, T(0)), T(1)));
         T CD = c.C_smag * a.Delta;
-        T nu0 = normS * (CD * CD) + T(1e-5);
+        T nu0 = c.learned ? gf[19] : normS * (CD * CD) + T(1e-5);
         T dotk = (nu0 * k[0] + nu0 * k[1]) + nu0 * k[2];
         T nu[3];
         for (int d = 0; d < 3; ++d) {
@@ -197,7 +233,7 @@ template <class T> inline void gradient_argument(const Ctx<T> &c, const T *q, co
     TS<T> ts = thermo(c, q, a.Phi);
     T e_tot = q[4] * (T(1) / q[0]);
     G[3] = e_tot + c.R_d * ts.temp;
-    G[4] = (c.d.turbulence == CMDG_SMAGORINSKY_LILLY) ? a.theta_v : T(0);
+    G[4] = c.smag ? a.theta_v : T(0);
 }
 
 // compute_gradient_flux!: energy.jl:50-59, TurbulenceClosures.jl:366-376,
@@ -212,8 +248,11 @@ template <class T> inline void gradient_flux(const Ctx<T> &c, const T gG[3][5], 
     gf[6] = gG[1][1];
     gf[7] = (gG[2][1] + gG[1][2]) / T(2);
     gf[8] = gG[2][2];
-    if (c.d.turbulence == CMDG_SMAGORINSKY_LILLY)
+    if (c.smag)
         gf[9] = ((gG[0][4] * a.gPhi[0] + gG[1][4] * a.gPhi[1]) + gG[2][4] * a.gPhi[2]) / a.theta_v;
+    if (c.learned)  // the velocity gradient itself, for the network (nonlinear: applied once assembled)
+        for (int cc = 0; cc < 3; ++cc)
+            for (int d = 0; d < 3; ++d) gf[10 + 3 * cc + d] = gG[d][cc];
 }
 
 // AtmosModel.jl:776-796
@@ -265,7 +304,7 @@ template <class T> void volume_gradients(Ctx<T> &c, const T *Q, const T *aux) {
             for (int j = 0; j < Nq; ++j)
                 for (int i = 0; i < Nq; ++i) {
                     const int n = i + Nq * (j + Nq * k);
-                    T gG[3][5] = {}, gf[10];
+                    T gG[3][5] = {}, gf[20];
                     for (int s = 0; s < ng; ++s) {
                         T g1 = 0, g2 = 0;
                         for (int m = 0; m < Nq; ++m) {
@@ -280,14 +319,14 @@ template <class T> void volume_gradients(Ctx<T> &c, const T *Q, const T *aux) {
                         }
                     }
                     gradient_flux(c, gG, A[n], gf);
-                    for (int s = 0; s < ngf; ++s) c.gradflux[n + (int64_t)Np * (s + (int64_t)ngf * e)] = gf[s];
+                    for (int s = 0; s < c.ngf_lin; ++s) c.gradflux[n + (int64_t)Np * (s + (int64_t)ngf * e)] = gf[s];
                     for (int s = 0; s < ng; ++s) {
                         T g3 = -T(0);
                         for (int m = 0; m < Nq; ++m) g3 += c.D[k + Nq * m] * G[i + Nq * (j + Nq * m)][s];
                         for (int d = 0; d < 3; ++d) gG[d][s] = -T(0) + vg(c, n, 2 + 3 * d, e) * g3;
                     }
                     gradient_flux(c, gG, A[n], gf);
-                    for (int s = 0; s < ngf; ++s) c.gradflux[n + (int64_t)Np * (s + (int64_t)ngf * e)] += gf[s];
+                    for (int s = 0; s < c.ngf_lin; ++s) c.gradflux[n + (int64_t)Np * (s + (int64_t)ngf * e)] += gf[s];
                 }
     }
 }
@@ -346,12 +385,32 @@ template <class T> void interface_gradients(Ctx<T> &c, const T *Q, const T *aux)
                 }
                 for (int d = 0; d < 3; ++d)
                     for (int s = 0; s < ng; ++s) nG[d][s] = nv[d] * Gm[s];
-                T gfs[10], gfm[10];
+                T gfs[20], gfm[20];
                 gradient_flux(c, Gs, am, gfs);
                 gradient_flux(c, nG, am, gfm);
-                for (int s = 0; s < ngf; ++s)
+                for (int s = 0; s < c.ngf_lin; ++s)
                     c.gradflux[x.vm + (int64_t)Np * (s + (int64_t)ngf * x.em)] += vMI * sM * (gfs[s] - gfm[s]);
             }
+}
+
+// nu_0 of the learned closure at every node, once the gradient is assembled.
+template <class T> void learned_viscosity(Ctx<T> &c, const T *aux) {
+    const int Np = c.Np, ngf = c.ngf;
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < c.nelem; ++e)
+        for (int n = 0; n < Np; ++n) {
+            T *gf = &c.gradflux[n + (int64_t)Np * (int64_t)ngf * e];
+            auto at = [&](int s) -> T & { return gf[(int64_t)Np * s]; };
+            NodeAux<T> a = load_aux(c, aux, n, e);
+            double raw[10];
+            for (int s = 0; s < 9; ++s) raw[s] = (double)at(10 + s);
+            raw[9] = (double)a.Delta;
+            const double y = std::min(mlp_output(c, raw), 8.0);
+            const double S0 = at(3), S1 = at(4), S2 = at(5), S3 = at(6), S4 = at(7), S5 = at(8);
+            const double norm2 = S0 * S0 + S3 * S3 + S5 * S5 + 2.0 * (S1 * S1 + S2 * S2 + S4 * S4);
+            const double D = (double)a.Delta;
+            at(19) = (T)(std::exp(y) * (D * D) * std::sqrt(2.0 * norm2) + 1e-5);
+        }
 }
 
 // DGModel_kernels.jl:64-309 (horizontal pass, writes alpha*T + beta*tendency) and
@@ -364,7 +423,7 @@ template <class T> void volume_tendency(Ctx<T> &c, T *tend, const T *Q, const T 
         T Ft[NPMAX][3][5];
         T src[NPMAX][5];
         for (int n = 0; n < Np; ++n) {
-            T q[5], gf[10] = {};
+            T q[5], gf[20] = {};
             for (int s = 0; s < 5; ++s) q[s] = Q[n + (int64_t)Np * (s + 5 * e)];
             NodeAux<T> a = load_aux(c, aux, n, e);
             T F[3][5], F2[3][5];
@@ -447,7 +506,7 @@ template <class T> void interface_tendency(Ctx<T> &c, T *tend, const T *Q, const
                 T nv[3] = {sg(c, SN1, fn, f, e), sg(c, SN2, fn, f, e), sg(c, SN3, fn, f, e)};
                 T sM = sg(c, SSM, fn, f, e), vMI = sg(c, SVMI, fn, f, e);
                 FaceIdx x = face_idx(c, e, f, fn);
-                T qm[5], qp[5], gfm[10] = {}, gfp[10] = {};
+                T qm[5], qp[5], gfm[20] = {}, gfp[20] = {};
                 for (int s = 0; s < 5; ++s) {
                     qm[s] = Q[x.vm + (int64_t)Np * (s + 5 * x.em)];
                     qp[s] = Q[x.vp + (int64_t)Np * (s + 5 * x.ep)];
@@ -476,6 +535,7 @@ template <class T> void rhs(Ctx<T> &c, T *tend, const T *Q, T *aux, T alpha, T b
     if (c.diffusive) {
         volume_gradients(c, Q, aux);
         interface_gradients(c, Q, aux);
+        if (c.learned) learned_viscosity(c, aux);
     }
     volume_tendency(c, tend, Q, aux, alpha, beta);
     interface_tendency(c, tend, Q, aux, alpha);
@@ -614,8 +674,17 @@ template <class T> Ctx<T> *make_ctx(const cmdg_desc *d) {
     c->visc = (T)d->viscosity;
     c->C_smag = (T)d->C_smag;
     c->inv_Pr = (T)d->inv_Pr_turb;
-    c->ngrad = (d->turbulence == CMDG_SMAGORINSKY_LILLY) ? 5 : 4;
-    c->ngf = (d->turbulence == CMDG_SMAGORINSKY_LILLY) ? 10 : 9;
+    c->learned = d->turbulence == CMDG_LEARNED_CLOSURE;
+    c->smag = d->turbulence == CMDG_SMAGORINSKY_LILLY || c->learned;
+    c->ngrad = c->smag ? 5 : 4;
+    // learned: [grad h (3), S (6), N2, grad u (9), nu_0]
+    c->ngf = c->learned ? 20 : (c->smag ? 10 : 9);
+    c->ngf_lin = c->learned ? 19 : c->ngf;
+    for (double &w : c->w1) w = 0.0;
+    for (double &w : c->w2) w = 0.0;
+    for (int o = 0; o < 32; ++o) c->b1[o] = c->b2[o] = c->w3[o] = 0.0;
+    c->b3 = 0.0;
+    for (int k = 0; k < 10; ++k) { c->shift[k] = 0.0; c->scale[k] = 1.0; }
     c->diffusive = true;
     c->gradflux.assign((size_t)c->Np * c->ngf * E, T(0));
     return c;
@@ -698,6 +767,36 @@ int oracle_flux_second_order(oracle_ctx *o, const double *q, const double *gf, c
     flux_second_order(c, q, gf, a, F2);
     for (int d = 0; d < 3; ++d)
         for (int s = 0; s < 5; ++s) F[d * 5 + s] = F2[d][s];
+    return 0;
+}
+
+// Learned-closure network (BF16-valued weights passed as double; shift/scale FP32).
+int oracle_mlp_set(oracle_ctx *o, const double *w1, const double *b1, const double *w2, const double *b2,
+                   const double *w3, double b3, const double *shift, const double *scale) {
+    auto set = [&](auto &c) {
+        for (int q = 0; q < 320; ++q) c.w1[q] = bf16_round(w1[q]);
+        for (int q = 0; q < 1024; ++q) c.w2[q] = bf16_round(w2[q]);
+        for (int q = 0; q < 32; ++q) {
+            c.w3[q] = bf16_round(w3[q]);
+            c.b1[q] = (double)(float)b1[q];
+            c.b2[q] = (double)(float)b2[q];
+        }
+        c.b3 = (double)(float)b3;
+        for (int q = 0; q < 10; ++q) {
+            c.shift[q] = (double)(float)shift[q];
+            c.scale[q] = (double)(float)scale[q];
+        }
+    };
+    if (o->use_float) set(*(Ctx<float> *)o->impl);
+    else set(*(Ctx<double> *)o->impl);
+    return 0;
+}
+
+// Network output y for n raw input rows (row-major n x 10).
+int oracle_mlp_forward(oracle_ctx *o, const double *raw, int64_t n, double *y) {
+    if (o->use_float) return 1;
+    Ctx<double> &c = *(Ctx<double> *)o->impl;
+    for (int64_t i = 0; i < n; ++i) y[i] = mlp_output(c, raw + 10 * i);
     return 0;
 }
 

@@ -44,6 +44,8 @@ def load():
         lib.oracle_lsrk54_coefficient.restype = C.c_double
         lib.oracle_num_threads.restype = C.c_int
         lib.oracle_flux_second_order.argtypes = [vp, vp, vp, vp, C.c_double, vp]
+        lib.oracle_mlp_set.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp]
+        lib.oracle_mlp_forward.argtypes = [vp, vp, C.c_int64, vp]
         _lib = lib
     return _lib
 
@@ -92,8 +94,14 @@ class Oracle:
     def lsrk54_step(self, Q: np.ndarray, dQ: np.ndarray, t: float, dt: float):
         self.lib.oracle_lsrk54_step(self._ctx, self._arr(Q), self._arr(dQ), self._arr(self.aux), t, dt)
 
+    @property
+    def ngradflux(self) -> int:
+        # the learned closure also keeps the assembled velocity gradient (9) before nu_0
+        return 20 if self.cfg.turbulence == abi.LEARNED_CLOSURE else self.cfg.ngradflux
+
     def gradient_flux(self) -> np.ndarray:
-        out = np.zeros((self.grid.Np, self.cfg.ngradflux, self.grid.nelem), order="F")
+        """(Np, ngradflux, nelem); learned closure: [grad h, S, N2, grad u (9), nu_0]."""
+        out = np.zeros((self.grid.Np, self.ngradflux, self.grid.nelem), order="F")
         self.lib.oracle_gradient_flux(self._ctx, _p(out))
         return out
 
@@ -113,6 +121,20 @@ class Oracle:
         F = np.zeros(15)
         self.lib.oracle_flux_second_order(self._ctx, _p(q), _p(gf), _p(gp), float(Delta), _p(F))
         return F.reshape(3, 5)
+
+    def set_mlp(self, mlp):
+        """Load a closure.MLPParams network (weights rounded to BF16 inside)."""
+        f = mlp.flat.astype(np.float64)
+        parts = [np.ascontiguousarray(a) for a in (f[0:320], f[320:352], f[352:1376], f[1376:1408], f[1408:1440],
+                                                    mlp.shift.astype(np.float64), mlp.scale.astype(np.float64))]
+        w1, b1, w2, b2, w3, sh, sc = parts
+        self.lib.oracle_mlp_set(self._ctx, _p(w1), _p(b1), _p(w2), _p(b2), _p(w3), float(f[1440]), _p(sh), _p(sc))
+
+    def mlp_forward(self, raw: np.ndarray) -> np.ndarray:
+        raw = np.ascontiguousarray(raw, dtype=np.float64)
+        y = np.zeros(raw.shape[0])
+        self.lib.oracle_mlp_forward(self._ctx, _p(raw.ravel()), raw.shape[0], _p(y))
+        return y
 
     def pressure_gradient(self, p: np.ndarray) -> np.ndarray:
         out = np.zeros((self.grid.Np, 3, self.grid.nelem), order="F")
