@@ -264,10 +264,17 @@ def power_spectrum_3d(u, v, w, L: float, dim: int, nor: float = 1.0):
     return sums * 2 * np.pi * k ** 2 / (cnt * dx ** 3) / nor, k
 
 
-def spectrum_error(E, E_ref, kmax: Optional[int] = None) -> float:
-    """RMS over shells 1..kmax of log(E / E_ref)."""
+def spectrum_error(E, E_ref, kmax: Optional[int] = None, floor: float = 1e-8) -> dict:
+    """Errors of a spectrum against a reference over shells 1..kmax:
+    rel_l2 = |E - E_ref|_2 / |E_ref|_2, and log_rms = RMS of log(E / E_ref) over the
+    shells whose reference energy exceeds floor * max(E_ref) (shells holding only
+    roundoff are left out)."""
     n = len(E_ref) if kmax is None else kmax
-    return float(np.sqrt(np.mean(np.log(np.asarray(E[:n]) / np.asarray(E_ref[:n])) ** 2)))
+    E, R = np.asarray(E[:n], np.float64), np.asarray(E_ref[:n], np.float64)
+    keep = R > floor * R.max()
+    return dict(rel_l2=float(np.linalg.norm(E - R) / np.linalg.norm(R)),
+                log_rms=float(np.sqrt(np.mean(np.log(np.maximum(E[keep], 1e-300) / R[keep]) ** 2))),
+                shells=int(keep.sum()))
 
 
 # ----------------------------------------------------------------- training

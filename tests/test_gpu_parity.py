@@ -36,6 +36,16 @@ def _case(name):
         Q0[:, 0, :] *= 1 + 1e-4 * rng.uniform(-1, 1, Q0[:, 0, :].shape)
         Q0[:, 1:4, :] += 0.1 * rng.uniform(-1, 1, Q0[:, 1:4, :].shape)
         return cfg, grid, aux, Q0
+    if name == "tgv_kinematic":
+        # Taylor-Green vortex, constant kinematic viscosity with the divergence term
+        prm = atmos.ideal_gas_parameters()
+        grid = uniform_brick((3, 3, 3), (0.0, 0.0, 0.0), (2 * math.pi,) * 3, N=4)
+        cfg = atmos.AtmosConfig(prm, turbulence=abi.CONSTANT_KINEMATIC_VISCOSITY, viscosity=0.02,
+                                with_divergence=True)
+        aux = atmos.init_aux(cfg, grid)
+        Q0 = atmos.taylor_green(prm, grid, mach=0.1)
+        Q0[:, 1:4, :] *= 1 + 0.05 * rng.uniform(-1, 1, Q0[:, 1:4, :].shape)
+        return cfg, grid, aux, Q0
     # LES cases: walls in z, gravity, hydrostatic reference, Smagorinsky
     prm = atmos.ideal_gas_parameters()
     grid = uniform_brick((3, 3, 3), (0.0, 0.0, 0.0), (300.0, 300.0, 300.0), N=4,
@@ -53,7 +63,8 @@ def _case(name):
     return cfg, grid, aux, Q0
 
 
-CASES = ["euler_periodic", "euler_central", "viscous_periodic", "les_walls", "les_central", "les_no_subtract"]
+CASES = ["euler_periodic", "euler_central", "viscous_periodic", "tgv_kinematic", "les_walls", "les_central",
+         "les_no_subtract"]
 
 
 def _model(cfg, grid, aux, geometry):
@@ -112,13 +123,13 @@ def test_rhs_parity(name, geometry):
 
 
 @pytest.mark.parametrize("geometry", ["affine", "general"])
-@pytest.mark.parametrize("name", ["euler_periodic", "les_walls"])
+@pytest.mark.parametrize("name", ["euler_periodic", "tgv_kinematic", "les_walls"])
 def test_lsrk54_step_parity(name, geometry):
     import torch
     from cmdg.dg import LSRK54CarpenterKennedy, solve, to_device, to_host
     print(f"[parity] lsrk54 {name} {geometry}", flush=True)
     cfg, grid, aux, Q0 = _case(name)
-    dt = 1e-3 if name.startswith("euler") else 5e-3
+    dt = {"euler_periodic": 1e-3, "tgv_kinematic": 1e-4}.get(name, 5e-3)
     orc = O.Oracle(cfg, grid, aux)
     Qr = np.asfortranarray(Q0.copy())
     dQr = np.zeros_like(Qr, order="F")
