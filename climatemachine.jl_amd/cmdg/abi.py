@@ -70,7 +70,7 @@ EXPORTED = (
     "cmdg_lsrk_stage", "cmdg_lsrk_update", "cmdg_aux", "cmdg_gradient_flux",
     "cmdg_gradient_flux_f32", "cmdg_mlp_set_weights", "cmdg_mlp_forward", "cmdg_velocity_gradient",
     "cmdg_box_filter", "cmdg_closure_samples", "cmdg_mlp_loss_grad", "cmdg_mlp_adam", "cmdg_interpolate_equispaced",
-    "cmdg_geometry_kind", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
+    "cmdg_courant", "cmdg_weightedsum", "cmdg_geometry_kind", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
     "cmdg_last_error", "cmdg_version",
 )
 MLP_NPARAM = 1441
@@ -112,6 +112,8 @@ def load_library(path: str = None):
                                   vp]
     lib.cmdg_interpolate_equispaced.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, C.c_int64, C.c_int64,
                                                 C.c_int64, vp, vp]
+    lib.cmdg_courant.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
+    lib.cmdg_weightedsum.argtypes = [vp, vp, C.c_int32, C.POINTER(C.c_double)]
     lib.cmdg_geometry_kind.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.cmdg_set_kernel_timing.argtypes = [vp, C.c_int]
     lib.cmdg_kernel_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]

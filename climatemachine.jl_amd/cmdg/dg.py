@@ -173,6 +173,23 @@ class DGModel:
                                              float(t_stage), float(dt), float(rka_next), float(rkb)),
                     "cmdg_lsrk_stage")
 
+    def courant(self, Q, dt: float, kind: str = "nondiffusive", direction: str = "every") -> float:
+        """courant(local_courant, dg, m, Q, dt, t, direction): the largest local
+        Courant number; kind advective / nondiffusive / diffusive."""
+        k = {"advective": 0, "nondiffusive": 1, "diffusive": 2}[kind]
+        d = {"every": 0, "horizontal": 1, "vertical": 2}[direction]
+        out = C.c_double()
+        self._check(self.lib.cmdg_courant(self._ctx, C.c_void_p(_ptr(Q)), float(dt), k, d, C.byref(out)),
+                    "cmdg_courant")
+        return out.value
+
+    def weightedsum(self, Q, state: int) -> float:
+        """weightedsum(Q, state): sum of M * Q over all nodes (double-double)."""
+        out = C.c_double()
+        self._check(self.lib.cmdg_weightedsum(self._ctx, C.c_void_p(_ptr(Q)), int(state), C.byref(out)),
+                    "cmdg_weightedsum")
+        return out.value
+
     @property
     def geometry_kind(self) -> str:
         k = C.c_int32()

@@ -38,6 +38,7 @@
 #include "cmdg_affine.h"
 #include "cmdg_mixed.h"
 #include "cmdg_learn.h"
+#include "cmdg_diag.h"
 
 namespace {
 using namespace cmdg;
@@ -92,6 +93,11 @@ struct cmdg_ctx {
     // learned closure (mixed precision only)
     bool learned = false, mlp_ready = false;
     MlpDev *d_mlp = nullptr;
+    // diagnostics (Courant numbers, weighted sums): reduction scratch, allocated on first use
+    int32_t aux_coord = -1;
+    double2 *d_part = nullptr;
+    double *d_scalar = nullptr;
+    int64_t npart = 0;
     std::string err;
     // optional per-kernel event timing (cmdg_set_kernel_timing)
     bool timing = false;
@@ -393,6 +399,7 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     c->nelem = d->nelem;
     if ((double)c->nelem * c->Np >= 2147483647.0) return fail(c, CMDG_EUNSUPPORTED, "mesh too large for int32 node ids");
     c->learned = d->turbulence == CMDG_LEARNED_CLOSURE;
+    c->aux_coord = d->aux_coord;
     c->smag = d->turbulence == CMDG_SMAGORINSKY_LILLY || c->learned;
     c->diffusive = c->smag || d->viscosity != 0.0;
     c->ngf = c->learned ? 11 : (c->smag ? 10 : 9);
@@ -552,6 +559,8 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_gf32);
     (void)hipFree(c->d_f2tr32);
     (void)hipFree(c->d_mlp);
+    (void)hipFree(c->d_part);
+    (void)hipFree(c->d_scalar);
     delete c;
     return CMDG_OK;
 }
@@ -768,6 +777,73 @@ int cmdg_interpolate_equispaced(const double *Q, const int32_t *eijk, int64_t ne
         hipLaunchKernelGGL(k_interp_equispaced<4>, dim3((unsigned)nelem), dim3(nt < 64 ? 64 : nt), 0,
                            (hipStream_t)stream, Q, eijk, nelem, p, IM, nx, ny, nz, u);
     return hipGetLastError() == hipSuccess ? CMDG_OK : CMDG_EHIP;
+}
+
+static int diag_scratch(cmdg_ctx *c, int64_t n) {
+    if (!c->d_scalar) {
+        int rc = check_hip(c, hipMalloc((void **)&c->d_scalar, sizeof(double)), "hipMalloc");
+        if (rc) return rc;
+    }
+    if (c->npart < n) {
+        (void)hipFree(c->d_part);
+        c->d_part = nullptr;
+        c->npart = 0;
+        int rc = check_hip(c, hipMalloc((void **)&c->d_part, (size_t)n * sizeof(double2)), "hipMalloc");
+        if (rc) return rc;
+        c->npart = n;
+    }
+    return CMDG_OK;
+}
+
+static int diag_result(cmdg_ctx *c, double *out) {
+    int rc = check_hip(c, hipGetLastError(), "diagnostic launch");
+    if (rc) return rc;
+    rc = check_hip(c, hipMemcpyAsync(out, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    if (rc) return rc;
+    return check_hip(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+int cmdg_courant(cmdg_ctx *c, const double *Q, double dt, int32_t kind, int32_t direction, double *out) {
+    if (!c) return CMDG_EINVAL;
+    if (!Q || !out) return fail(c, CMDG_EINVAL, "null pointer");
+    if (kind < 0 || kind > 2 || direction < 0 || direction > 2) return fail(c, CMDG_EINVAL, "bad Courant kind or direction");
+    if (c->aux_coord < 0) return fail(c, CMDG_EINVAL, "the Courant number needs coordinates in aux");
+    if (kind == COURANT_DIFFUSIVE && c->mixed)
+        return fail(c, CMDG_EUNSUPPORTED, "the diffusive Courant number reads the FP64 gradient flux");
+    int rc = diag_scratch(c, c->nelem);
+    if (rc) return rc;
+    double *part = reinterpret_cast<double *>(c->d_part);
+    const double *gf = (kind == COURANT_DIFFUSIVE && c->diffusive) ? c->d_gf : nullptr;
+    const int ngf = gf ? c->ngf : 0;
+    dim3 grid((unsigned)c->nelem);
+#define CMDG_COURANT(NQ, SM)                                                                                     \
+    hipLaunchKernelGGL((k_courant<NQ, SM>), grid, dim3(threads_for<NQ>()), 0, c->stream, c->P, Q, c->d_aux, gf, \
+                       ngf, c->nelem, dt, kind, direction, c->aux_coord, part)
+    if (c->Nq == 5) {
+        if (c->smag) CMDG_COURANT(5, true);
+        else CMDG_COURANT(5, false);
+    } else {
+        if (c->smag) CMDG_COURANT(4, true);
+        else CMDG_COURANT(4, false);
+    }
+#undef CMDG_COURANT
+    hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, c->stream, part, c->nelem, c->d_scalar);
+    return diag_result(c, out);
+}
+
+int cmdg_weightedsum(cmdg_ctx *c, const double *Q, int32_t state, double *out) {
+    if (!c) return CMDG_EINVAL;
+    if (!Q || !out) return fail(c, CMDG_EINVAL, "null pointer");
+    if (state < 0 || state > 4) return fail(c, CMDG_EINVAL, "state index out of range");
+    const int64_t total = (int64_t)c->Np * c->nelem;
+    int64_t nb = (total + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    int rc = diag_scratch(c, nb);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_weightedsum, dim3((unsigned)nb), dim3(256), 0, c->stream, Q, c->d_vg, c->Np, NVG, state,
+                       c->nelem, c->d_part);
+    hipLaunchKernelGGL(k_dd_reduce, dim3(1), dim3(256), 0, c->stream, c->d_part, (int)nb, c->d_scalar);
+    return diag_result(c, out);
 }
 
 int cmdg_geometry_kind(cmdg_ctx *c, int32_t *kind) {

@@ -208,6 +208,22 @@ int cmdg_interpolate_equispaced(const double *Q, const int32_t *eijk, int64_t ne
                                 int32_t p, const double *L, int64_t nx, int64_t ny, int64_t nz,
                                 double *u, void *stream);
 
+/* Largest local Courant number of Q over all nodes (courant with the Atmos local
+ * Courant functions: src/Numerics/DGMethods/SpaceDiscretization.jl:307-365,
+ * DGModel_kernels.jl:3028-3095, src/Atmos/Model/courant.jl:11-73), with the
+ * node spacing of kernel_min_neighbor_distance! (src/Numerics/Mesh/Grids.jl:1228-1334)
+ * taken from the aux coordinates.  kind: 0 advective dt|u|/dx, 1 nondiffusive
+ * dt(|u| + c)/dx, 2 diffusive dt nu/dx^2 (nu from the gradient flux of the last RHS,
+ * FP64 contexts); direction: 0 EveryDirection, 1 HorizontalDirection,
+ * 2 VerticalDirection.  Synchronises; *courant is a host double. */
+int cmdg_courant(cmdg_ctx *ctx, const double *Q, double dt, int32_t kind, int32_t direction,
+                 double *courant);
+
+/* sum over all nodes of M * Q[:, state, :] (M the mass-matrix diagonal, vgeo field
+ * M) accumulated in double-double: weightedsum(Q, state),
+ * src/Arrays/MPIStateArrays.jl:655-674.  Synchronises; *sum is a host double. */
+int cmdg_weightedsum(cmdg_ctx *ctx, const double *Q, int32_t state, double *sum);
+
 /* Geometry representation the context chose (diagnostic; no reference counterpart):
  * 1 = axis-aligned affine bricks (per-element metrics, detected bitwise at create),
  * 0 = general per-node metrics.  Setting CMDG_GENERAL_GEOMETRY=1 in the environment

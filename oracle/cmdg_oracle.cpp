@@ -648,6 +648,80 @@ template <class T> void pressure_gradient(Ctx<T> &c, const T *p, T *gp) {
     }
 }
 
+// ------------------------------------------------------------ diagnostics
+// courant (SpaceDiscretization.jl:307-365) with kernel_min_neighbor_distance!
+// (Grids.jl:1228-1334, node coordinates from vgeo x1..x3) and the Atmos local
+// Courant numbers (src/Atmos/Model/courant.jl:11-73); the gradient flux is the one of
+// the last RHS.  kind 0 advective, 1 nondiffusive, 2 diffusive; direction 0 every,
+// 1 horizontal, 2 vertical.
+template <class T> double courant(const Ctx<T> &c, const T *Q, const T *aux, double dt, int kind, int direction) {
+    const int Nq = c.Nq, Np = c.Np;
+    double best = -INFINITY;
+    for (int64_t e = 0; e < c.nelem; ++e)
+        for (int n = 0; n < Np; ++n) {
+            const int ijk[3] = {n % Nq, (n / Nq) % Nq, n / (Nq * Nq)}, stride[3] = {1, Nq, Nq * Nq};
+            const bool use[3] = {direction != 2, direction != 2, direction != 1};
+            double md = INFINITY;
+            for (int a = 0; a < 3; ++a) {
+                if (!use[a]) continue;
+                for (int s = -1; s <= 1; s += 2) {
+                    const int m = ijk[a] + s;
+                    if (m < 0 || m >= Nq) continue;
+                    const int o = n + s * stride[a];
+                    double d2 = 0;
+                    for (int x = 0; x < 3; ++x) {
+                        const double dx = (double)vg(c, n, VX1 + x, e) - (double)vg(c, o, VX1 + x, e);
+                        d2 += dx * dx;
+                    }
+                    md = std::min(md, std::sqrt(d2));
+                }
+            }
+            T q[5];
+            for (int s = 0; s < 5; ++s) q[s] = Q[n + (int64_t)Np * (s + 5 * e)];
+            NodeAux<T> a = load_aux(c, aux, n, e);
+            double kh[3];
+            for (int d = 0; d < 3; ++d) kh[d] = (double)(a.gPhi[d] / c.grav);
+            double val;
+            if (kind == 2) {
+                double nn = 0;
+                if (c.diffusive && (c.smag || c.visc != T(0))) {
+                    const T *gf = &c.gradflux[n + (int64_t)Np * (int64_t)c.ngf * e];
+                    T g[20] = {};
+                    for (int s = 0; s < c.ngf; ++s) g[s] = gf[(int64_t)Np * s];
+                    if (!c.smag) {
+                        nn = (double)((c.d.turbulence == CMDG_CONSTANT_DYNAMIC_VISCOSITY) ? c.visc / q[0] : c.visc);
+                    } else {
+                        Turb<T> tb = turbulence(c, q, g, a);
+                        double nu[3];
+                        for (int d = 0; d < 3; ++d) nu[d] = (double)tb.Dt[d] / (double)c.inv_Pr;
+                        const double dk = nu[0] * kh[0] + nu[1] * kh[1] + nu[2] * kh[2];
+                        if (direction == 2) nn = dk;
+                        else {
+                            if (direction == 1)
+                                for (int d = 0; d < 3; ++d) nu[d] -= dk * kh[d];
+                            nn = std::sqrt(nu[0] * nu[0] + nu[1] * nu[1] + nu[2] * nu[2]);
+                        }
+                    }
+                }
+                val = dt * nn / (md * md);
+            } else {
+                double m3[3] = {(double)q[1], (double)q[2], (double)q[3]};
+                const double mk = m3[0] * kh[0] + m3[1] * kh[1] + m3[2] * kh[2];
+                double nu;
+                if (direction == 2) nu = std::fabs(mk) / (double)q[0];
+                else {
+                    if (direction == 1)
+                        for (int d = 0; d < 3; ++d) m3[d] -= mk * kh[d];
+                    nu = std::sqrt(m3[0] * m3[0] + m3[1] * m3[1] + m3[2] * m3[2]) / (double)q[0];
+                }
+                if (kind == 1) nu += (double)soundspeed(c, thermo(c, q, a.Phi));
+                val = dt * nu / md;
+            }
+            best = std::max(best, val);
+        }
+    return best;
+}
+
 template <class T> Ctx<T> *make_ctx(const cmdg_desc *d) {
     Ctx<T> *c = new Ctx<T>();
     c->d = *d;
@@ -798,6 +872,23 @@ int oracle_mlp_forward(oracle_ctx *o, const double *raw, int64_t n, double *y) {
     Ctx<double> &c = *(Ctx<double> *)o->impl;
     for (int64_t i = 0; i < n; ++i) y[i] = mlp_output(c, raw + 10 * i);
     return 0;
+}
+
+// Largest local Courant number (double contexts).
+double oracle_courant(oracle_ctx *o, const double *Q, const double *aux, double dt, int kind, int direction) {
+    if (o->use_float) return NAN;
+    return courant(*(Ctx<double> *)o->impl, Q, aux, dt, kind, direction);
+}
+
+// sum of M * Q[:, state, :] in long double (the reference accumulates in double-double)
+double oracle_weightedsum(oracle_ctx *o, const double *Q, int state) {
+    if (o->use_float) return NAN;
+    Ctx<double> &c = *(Ctx<double> *)o->impl;
+    long double acc = 0.0L;
+    for (int64_t e = 0; e < c.nelem; ++e)
+        for (int n = 0; n < c.Np; ++n)
+            acc += (long double)Q[n + (int64_t)c.Np * (state + 5 * e)] * (long double)vg(c, n, VM, e);
+    return (double)acc;
 }
 
 // LSRK54 coefficient k of table which (0 = A, 1 = B, 2 = C) in precision prec.

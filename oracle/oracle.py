@@ -46,6 +46,10 @@ def load():
         lib.oracle_flux_second_order.argtypes = [vp, vp, vp, vp, C.c_double, vp]
         lib.oracle_mlp_set.argtypes = [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp]
         lib.oracle_mlp_forward.argtypes = [vp, vp, C.c_int64, vp]
+        lib.oracle_courant.argtypes = [vp, vp, vp, C.c_double, C.c_int, C.c_int]
+        lib.oracle_courant.restype = C.c_double
+        lib.oracle_weightedsum.argtypes = [vp, vp, C.c_int]
+        lib.oracle_weightedsum.restype = C.c_double
         _lib = lib
     return _lib
 
@@ -121,6 +125,16 @@ class Oracle:
         F = np.zeros(15)
         self.lib.oracle_flux_second_order(self._ctx, _p(q), _p(gf), _p(gp), float(Delta), _p(F))
         return F.reshape(3, 5)
+
+    def courant(self, Q, dt, kind=1, direction=0) -> float:
+        """Largest local Courant number (kind 0 advective, 1 nondiffusive, 2 diffusive
+        with the gradient flux of the last RHS; direction 0 every, 1 horizontal, 2 vertical)."""
+        Qf = np.asfortranarray(Q, dtype=np.float64)
+        return self.lib.oracle_courant(self._ctx, _p(Qf), _p(self.aux), float(dt), int(kind), int(direction))
+
+    def weightedsum(self, Q, state: int) -> float:
+        Qf = np.asfortranarray(Q, dtype=np.float64)
+        return self.lib.oracle_weightedsum(self._ctx, _p(Qf), int(state))
 
     def set_mlp(self, mlp):
         """Load a closure.MLPParams network (weights rounded to BF16 inside)."""

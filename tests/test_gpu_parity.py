@@ -251,3 +251,26 @@ def test_mixed_lsrk_steps_error():
     print("  rel L2 of the change: " + " ".join(f"{r:.2e}" for r in errs), flush=True)
     for r in errs:
         assert r < 1e-5   # measured max 2.9e-7 (energy)
+
+
+@pytest.mark.parametrize("name", ["les_walls", "tgv_kinematic"])
+def test_courant_and_weightedsum(name):
+    """courant (every kind and direction) and weightedsum against the oracle."""
+    import torch
+    from cmdg.dg import to_device
+    print(f"[parity] courant/weightedsum {name}", flush=True)
+    cfg, grid, aux, Q0 = _case(name)
+    orc = O.Oracle(cfg, grid, aux)
+    orc.tendency(Q0)
+    dg = _model(cfg, grid, aux, "affine")
+    Q = to_device(Q0)
+    dg(torch.zeros_like(Q), Q, None, 0.0)
+    for kind, k in (("advective", 0), ("nondiffusive", 1), ("diffusive", 2)):
+        for direction, d in (("every", 0), ("horizontal", 1), ("vertical", 2)):
+            got = dg.courant(Q, 0.01, kind, direction)
+            ref = orc.courant(Q0, 0.01, k, d)
+            assert abs(got - ref) <= 1e-13 * abs(ref), (kind, direction, got, ref)
+    for s in range(5):
+        got, ref = dg.weightedsum(Q, s), orc.weightedsum(Q0, s)
+        assert abs(got - ref) <= 2e-16 * max(abs(ref), 1e-300) + 1e-300, (s, got, ref)
+    dg.close()

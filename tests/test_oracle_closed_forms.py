@@ -211,3 +211,61 @@ def test_smagorinsky_lilly_stratification(Ri, f):
     nu0 = (0.2 * Delta) ** 2 * abs(a) + 1e-5
     assert F[0, 3] == pytest.approx(-nu0 * a * rho, rel=1e-13)              # horizontal row: nu0
     assert F[2, 1] == pytest.approx(-nu0 * f * a * rho, rel=1e-13, abs=1e-300)  # vertical row: nu0 f
+
+
+# ---------------------------------------------------------------- diagnostics
+def _uniform_flow_case(flat=False):
+    import math
+    from cmdg import abi, atmos
+    from cmdg.grid import uniform_brick
+    prm = atmos.ideal_gas_parameters()
+    grid = uniform_brick((2, 3, 2), (0.0, 0.0, 0.0), (200.0, 300.0, 100.0), N=4,
+                         periodicity=(True, True, False), boundary=((0, 0), (0, 0), (1, 2)))
+    kw = dict(orientation=abi.FLAT_ORIENTATION) if flat else {}
+    cfg = atmos.AtmosConfig(prm, turbulence=abi.CONSTANT_KINEMATIC_VISCOSITY, viscosity=3.0, **kw)
+    aux = atmos.init_aux(cfg, grid)
+    u0, T = 7.0, 290.0
+    Q = np.zeros((grid.Np, 5, grid.nelem), order="F")
+    Q[:, 0] = 1.1
+    Q[:, 1] = 1.1 * u0
+    Phi = aux[:, cfg.layout["Phi"]] if flat else 0.0
+    Q[:, 4] = 1.1 * (prm.cv_d * (T - prm.T_0) + 0.5 * u0 ** 2 + Phi)
+    return prm, grid, cfg, aux, Q, u0, T
+
+
+def test_courant_closed_forms():
+    """Uniform flow u0 e_x at constant T: dt (u0 + c) / dx_min, dt u0 / dx_min and
+    dt nu / dx_min^2, dx_min the smallest LGL gap of the element (Grids.jl:1228-1334)."""
+    import math
+    from oracle import oracle as O
+    prm, grid, cfg, aux, Q, u0, T = _uniform_flow_case()
+    orc = O.Oracle(cfg, grid, aux)
+    orc.tendency(Q)
+    dt = 0.01
+    c = math.sqrt(prm.cp_d / prm.cv_d * prm.R_d * T)
+    xi = grid.xi
+    gap = np.diff(xi).min() / 2
+    dx = min(200.0 / 2, 300.0 / 3, 100.0 / 2) * gap
+    assert abs(orc.courant(Q, dt, 1, 0) - dt * (u0 + c) / dx) < 1e-12 * dt * (u0 + c) / dx
+    assert abs(orc.courant(Q, dt, 0, 0) - dt * u0 / dx) < 1e-12 * dt * u0 / dx
+    assert abs(orc.courant(Q, dt, 2, 0) - dt * 3.0 / dx ** 2) < 1e-12 * dt * 3.0 / dx ** 2
+    # FlatOrientation: the vertical advective number of a horizontal flow vanishes,
+    # the horizontal one uses only the horizontal node spacing
+    prm, grid, cfg, aux, Q, u0, T = _uniform_flow_case(flat=True)
+    orc = O.Oracle(cfg, grid, aux)
+    assert abs(orc.courant(Q, dt, 0, 2)) < 1e-15
+    dxh = min(200.0 / 2, 300.0 / 3) * gap
+    assert abs(orc.courant(Q, dt, 0, 1) - dt * u0 / dxh) < 1e-12 * dt * u0 / dxh
+
+
+def test_weightedsum_closed_forms():
+    """sum M q is the LGL quadrature: exact for polynomials of degree <= 2N - 1."""
+    from oracle import oracle as O
+    prm, grid, cfg, aux, Q, u0, T = _uniform_flow_case()
+    orc = O.Oracle(cfg, grid, aux)
+    assert abs(orc.weightedsum(Q, 0) - 1.1 * 200 * 300 * 100) < 1e-14 * 6.6e6
+    x, y, z = grid.coords
+    Q2 = Q.copy(order="F")
+    Q2[:, 2] = x ** 3 * y ** 2 + z
+    exact = (200 ** 4 / 4) * (300 ** 3 / 3) * 100 + 200 * 300 * 100 ** 2 / 2
+    assert abs(orc.weightedsum(Q2, 2) - exact) < 1e-13 * exact
