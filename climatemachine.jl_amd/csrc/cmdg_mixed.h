@@ -141,9 +141,12 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     static_assert(!LEARN || SMAG, "the learned closure keeps the Smagorinsky-Lilly structure");
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     constexpr int NG = SMAG ? 5 : 4, NGF = LEARN ? 11 : (SMAG ? 10 : 9);
+    constexpr int NT = threads_for<Nq>();
     __shared__ float sD[Nq * Nq];
     __shared__ float sG[NG][Np];
-    __shared__ uint16_t sX[LEARN ? (threads_for<Nq>() / 64) * MLP_LDS_PER_WAVE : 1];
+    __shared__ float sJ[6 * Nfp][5];
+    __shared__ float sF2[3][TR_F2][Np];
+    __shared__ uint16_t sX[LEARN ? (NT / 64) * MLP_LDS_PER_WAVE : 1];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     const int n = threadIdx.x;
@@ -177,37 +180,15 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         for (int s = 0; s < NG; ++s) sG[s][n] = G[s];
     }
     __syncthreads();
-    if (!LEARN && !act) return;
-
-    const float xi0 = (float)ge[0], xi1 = (float)ge[1], xi2 = (float)ge[2];
-    float g[3][5] = {};
-    int fl[3], fnl[3], nf = 0;
-    float gf[11];
-    if (act) {
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
-        if (s < NG) {
-#pragma unroll
-            for (int m = 0; m < Nq; ++m) {
-                g1 += sD[i + Nq * m] * sG[s][m + Nq * (j + Nq * k)];
-                g2 += sD[j + Nq * m] * sG[s][i + Nq * (m + Nq * k)];
-                g3 += sD[k + Nq * m] * sG[s][i + Nq * (j + Nq * m)];
-            }
-        }
-        g[0][s] = xi0 * g1;
-        g[1][s] = xi1 * g2;
-        g[2][s] = xi2 * g3;
-    }
-    // lifted central jumps, accumulated into the gradient: M^-1 Mf n (G* - G-)
-    const float MI = (float)(1.0 / (ge[3] * A.W[n]));
-    nf = node_faces<Nq>(i, j, k, fl, fnl);
-    for (int c = 0; c < nf; ++c) {
-        const int f = fl[c], d = f >> 1, fn = fnl[c];
+    // lifted central jumps n (G* - G-) of all face nodes, spread over every lane
+    for (int t = n; t < 6 * Nfp; t += NT) {
+        const int f = t / Nfp, fn = t % Nfp, d = f >> 1, vn = face_node<Nq>(f, fn);
         const float nd = (f & 1) ? 1.0f : -1.0f;
-        const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
         const int32_t ne = A.nbe[6 * e + f];
-        float Gp[5], jump[5];
+        float Gm[5], Gp[5];
+#pragma unroll
+        for (int s = 0; s < 5; ++s) Gm[s] = s < NG ? sG[s][vn] : 0.0f;
+        float jump[5];
         if (ne >= 0) {
             const int fo = f ^ 1;
             double qp[5];
@@ -216,18 +197,51 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
             const double Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
             grad_arg_f<SMAG>(P, MP, qp, thermo_d(P, qp, Php), Gp);
 #pragma unroll
-            for (int s = 0; s < 5; ++s) jump[s] = 0.5f * (Gp[s] - G[s]);
+            for (int s = 0; s < 5; ++s) jump[s] = nd * (0.5f * (Gp[s] - Gm[s]));
         } else {
-            double qb[5] = {q[0], q[1], q[2], q[3], q[4]};
-            qb[1 + d] = 0.0;  // the gradient wall state removes the normal momentum
-            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, Phi), Gp);
+            double qb[5];
 #pragma unroll
-            for (int s = 0; s < 5; ++s) jump[s] = Gp[s] - G[s];
+            for (int s = 0; s < 5; ++s) qb[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
+            qb[1 + d] = 0.0;  // the gradient wall state removes the normal momentum
+            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, A.str[tri<Nfp>(e, f, 0, TR_S, fn)]), Gp);
+#pragma unroll
+            for (int s = 0; s < 5; ++s) jump[s] = nd * (Gp[s] - Gm[s]);
         }
 #pragma unroll
-        for (int s = 0; s < 5; ++s) g[d][s] += (w * nd) * jump[s];
+        for (int s = 0; s < 5; ++s) sJ[t][s] = jump[s];
     }
-    grad_flux_f<SMAG>(g, gPhi, thv, gf);
+    __syncthreads();
+
+    const float xi0 = (float)ge[0], xi1 = (float)ge[1], xi2 = (float)ge[2];
+    float g[3][5] = {};
+    int fl[3], fnl[3], nf = 0;
+    float gf[11];
+    if (act) {
+#pragma unroll
+        for (int s = 0; s < 5; ++s) {
+            float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
+            if (s < NG) {
+#pragma unroll
+                for (int m = 0; m < Nq; ++m) {
+                    g1 += sD[i + Nq * m] * sG[s][m + Nq * (j + Nq * k)];
+                    g2 += sD[j + Nq * m] * sG[s][i + Nq * (m + Nq * k)];
+                    g3 += sD[k + Nq * m] * sG[s][i + Nq * (j + Nq * m)];
+                }
+            }
+            g[0][s] = xi0 * g1;
+            g[1][s] = xi1 * g2;
+            g[2][s] = xi2 * g3;
+        }
+        // lifted central jumps of this node's faces: M^-1 Mf n (G* - G-)
+        const float MI = (float)(1.0 / (ge[3] * A.W[n]));
+        nf = node_faces<Nq>(i, j, k, fl, fnl);
+        for (int c = 0; c < nf; ++c) {
+            const int f = fl[c], d = f >> 1, fn = fnl[c];
+            const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
+#pragma unroll
+            for (int s = 0; s < 5; ++s) g[d][s] += w * sJ[f * Nfp + fn][s];
+        }
+        grad_flux_f<SMAG>(g, gPhi, thv, gf);
     }
     if (LEARN) {
         // the network runs on the assembled gradient (volume + lifted face jumps):
@@ -239,27 +253,97 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
             for (int c = 0; c < MLP_IN; ++c) x[c] = 0.0f;
         const float y = mlp_wave(mlp, x, sX);
         gf[10] = mlp_nu0(y, (float)ge[10], strain_norm_f(gf));
-        if (!act) return;
     }
-    float *go = gfo + (int64_t)Np * NGF * e + n;
+    if (act) {
+        float *go = gfo + (int64_t)Np * NGF * e + n;
 #pragma unroll
-    for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
-
-    if (nf > 0) {
-        float nu[3], qf[5];
+        for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
+        // second-order flux rows of the face nodes, staged by direction
+        if (nf > 0) {
+            float nu[3], qf[5];
 #pragma unroll
-        for (int s = 0; s < 5; ++s) qf[s] = (float)q[s];
-        viscosity_f<SMAG, LEARN>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
-        for (int c = 0; c < nf; ++c) {
-            float F2[5];
-            flux2_row_f<SMAG>(MP, qf, gf, nu, fl[c] >> 1, F2);
+            for (int s = 0; s < 5; ++s) qf[s] = (float)q[s];
+            viscosity_f<SMAG, LEARN>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
+            for (int c = 0; c < nf; ++c) {
+                float F2[5];
+                flux2_row_f<SMAG>(MP, qf, gf, nu, fl[c] >> 1, F2);
 #pragma unroll
-            for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, fl[c], v, TR_F2, fnl[c])] = F2[1 + v];
+                for (int v = 0; v < TR_F2; ++v) sF2[fl[c] >> 1][v][n] = F2[1 + v];
+            }
         }
+    }
+    __syncthreads();
+    // ... written to the trace face-major
+    for (int t = n; t < 6 * Nfp; t += NT) {
+        const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
+#pragma unroll
+        for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = sF2[f >> 1][v][vn];
     }
 }
 
 // ------------------------------------------------------------------ tendency (mixed)
+// Numerical flux at face node fn of face f of element e (FP32 fluxes from the FP64
+// face traces; the jump q- - q+ and the pressure perturbation in FP64).
+template <int Nq, bool DIFF>
+__device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
+                                              int f, int fn, const double *__restrict__ qtr,
+                                              const float *__restrict__ f2tr, double p_e, float flux[5]) {
+#pragma clang fp contract(fast)
+    constexpr int Nfp = Nq * Nq;
+    const int d = f >> 1;
+    const float h = (f & 1) ? 0.5f : -0.5f;
+    const int32_t ne = A.nbe[6 * e + f];
+    const bool wall = ne < 0;
+    const int fo = f ^ 1;
+    double q[5], qp[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) q[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
+    const double Phi = A.str[tri<Nfp>(e, f, 0, TR_S, fn)], ref_p = A.str[tri<Nfp>(e, f, 1, TR_S, fn)];
+    double Php = Phi, refpp = ref_p;
+    if (!wall) {
+#pragma unroll
+        for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
+        Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+        refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
+    } else {
+#pragma unroll
+        for (int s = 0; s < 5; ++s) qp[s] = q[s];
+        qp[1 + d] = -q[1 + d];  // reflect the normal momentum
+    }
+    const ThermoD tm = thermo_d(P, q, Phi), tp = thermo_d(P, qp, Php);
+    float qf[5], qpf[5], uf[3], upf[3], jump[5];
+#pragma unroll
+    for (int s = 0; s < 5; ++s) {
+        qf[s] = (float)q[s];
+        qpf[s] = (float)qp[s];
+        jump[s] = (float)(q[s] - qp[s]);
+    }
+    const float rim = 1.0f / qf[0], rip = 1.0f / qpf[0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        uf[c] = qf[1 + c] * rim;
+        upf[c] = qpf[1 + c] * rip;
+    }
+    float Fm[5], Fp[5];
+    flux1_row_f(qf, uf, (float)tm.p, (float)(MP.ref_sub ? tm.p - ref_p : tm.p - p_e), d, Fm);
+    flux1_row_f(qpf, upf, (float)tp.p, (float)(MP.ref_sub ? tp.p - refpp : tp.p - p_e), d, Fp);
+#pragma unroll
+    for (int s = 0; s < 5; ++s) flux[s] = (Fm[s] + Fp[s]) * h;
+    if (MP.rusanov) {
+        const float umd = d == 0 ? uf[0] : (d == 1 ? uf[1] : uf[2]);
+        const float upd = d == 0 ? upf[0] : (d == 1 ? upf[1] : upf[2]);
+        const float mw = 0.5f * fmaxf(fabsf(umd) + sqrtf(MP.gammaR * (float)tm.temp),
+                                      fabsf(upd) + sqrtf(MP.gammaR * (float)tp.temp));
+#pragma unroll
+        for (int s = 0; s < 5; ++s) flux[s] += mw * jump[s];
+    }
+    if (DIFF && !wall) {
+#pragma unroll
+        for (int v = 0; v < TR_F2; ++v)
+            flux[1 + v] += (f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
+    }
+}
+
 template <int Nq, bool DIFF, bool SMAG, bool LEARN, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>())
 k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
@@ -268,9 +352,11 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
                double *__restrict__ Qout, double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
 
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
-    constexpr int NGF = LEARN ? 11 : (SMAG ? 10 : 9);
+    constexpr int NGF = LEARN ? 11 : (SMAG ? 10 : 9), NT = threads_for<Nq>();
+    static_assert(3 * 5 * sizeof(float) >= 5 * sizeof(double), "the state of the next stage is staged in sF");
     __shared__ float sD[Nq * Nq];
     __shared__ float sF[3][5][Np];
+    __shared__ float sFl[6 * Nfp][5];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     const int n = threadIdx.x;
@@ -280,6 +366,18 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     const double *ge = A.geo + (int64_t)AFF_GEO * e;
 
     double q[5], Phi = -0.0, ref_p = 0.0, p_e = 0.0;
+    if (!MP.ref_sub) {
+        // Without a subtracted reference pressure the momentum flux carries p
+        // relative to an element constant, the pressure at the element's first
+        // node.  A constant flux is annihilated exactly by the weak-form volume term
+        // plus this element's face terms (the LGL summation-by-parts identity on an
+        // affine element), so the offset only removes rounding: the FP32 flux holds
+        // the in-element pressure variation, not p itself.
+        const double *q0 = Q + (int64_t)Np * 5 * e;
+        const double qe[5] = {q0[0], q0[Np], q0[2 * Np], q0[3 * Np], q0[4 * Np]};
+        const double Phe = P.a_Phi >= 0 ? aux[(int64_t)Np * P.naux * e + (int64_t)Np * P.a_Phi] : -0.0;
+        p_e = thermo_d(P, qe, Phe).p;
+    }
     float qf[5], uf[3], gf[11], nu[3], gPhi[3] = {0.0f, 0.0f, 0.0f}, pf = 0.0f, pmf = 0.0f, cm = 0.0f, drho = 0.0f;
     float MI = 0.0f;
     if (act) {
@@ -300,18 +398,6 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
             double *aw = aux + (int64_t)Np * P.naux * e + n;
             if (P.a_theta_v >= 0) aw[(int64_t)Np * P.a_theta_v] = t.temp / pow(t.p / P.MSLP, P.kappa);
             if (P.a_air_T >= 0) aw[(int64_t)Np * P.a_air_T] = t.temp;
-        }
-        if (!MP.ref_sub) {
-            // Without a subtracted reference pressure the momentum flux carries p
-            // relative to an element constant, the pressure at the element's first
-            // node.  A constant flux is annihilated exactly by the weak-form volume
-            // term plus this element's face terms (the LGL summation-by-parts
-            // identity on an affine element), so the offset only removes rounding:
-            // the FP32 flux holds the in-element pressure variation, not p itself.
-            const double *q0 = Q + (int64_t)Np * 5 * e;
-            const double qe[5] = {q0[0], q0[Np], q0[2 * Np], q0[3 * Np], q0[4 * Np]};
-            const double Phe = P.a_Phi >= 0 ? aux[(int64_t)Np * P.naux * e + (int64_t)Np * P.a_Phi] : -0.0;
-            p_e = thermo_d(P, qe, Phe).p;
         }
         pf = (float)t.p;
         pmf = (float)(MP.ref_sub ? t.p - ref_p : t.p - p_e);
@@ -347,96 +433,70 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         }
     }
     __syncthreads();
-    if (!act) return;
+    // numerical fluxes of all face nodes, spread over every lane
+    for (int t = n; t < 6 * Nfp; t += NT) {
+        float fl5[5];
+        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, qtr, f2tr, p_e, fl5);
+#pragma unroll
+        for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
+    }
+    __syncthreads();
 
-    float T[5];
+    double qn[5];
+    if (act) {
+        float T[5];
 #pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        float acc = 0.0f;
+        for (int s = 0; s < 5; ++s) {
+            float acc = 0.0f;
 #pragma unroll
-        for (int m = 0; m < Nq; ++m) {
-            acc += sD[m + Nq * i] * sF[0][s][m + Nq * (j + Nq * k)];
-            acc += sD[m + Nq * j] * sF[1][s][i + Nq * (m + Nq * k)];
-            acc += sD[m + Nq * k] * sF[2][s][i + Nq * (j + Nq * m)];
+            for (int m = 0; m < Nq; ++m) {
+                acc += sD[m + Nq * i] * sF[0][s][m + Nq * (j + Nq * k)];
+                acc += sD[m + Nq * j] * sF[1][s][i + Nq * (m + Nq * k)];
+                acc += sD[m + Nq * k] * sF[2][s][i + Nq * (j + Nq * m)];
+            }
+            T[s] = MI * acc;
         }
-        T[s] = MI * acc;
-    }
-    if (MP.gravity) {
+        if (MP.gravity) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) T[1 + d] -= drho * gPhi[d];
-    }
-
-    int fl[3], fnl[3];
-    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
-    for (int c = 0; c < nf; ++c) {
-        const int f = fl[c], d = f >> 1, fn = fnl[c];
-        const float h = (f & 1) ? 0.5f : -0.5f;
-        const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
-        const int32_t ne = A.nbe[6 * e + f];
-        const bool wall = ne < 0;
-        const int fo = f ^ 1;
-        double qp[5];
-        double Php = Phi, refpp = ref_p;
-        if (!wall) {
+            for (int d = 0; d < 3; ++d) T[1 + d] -= drho * gPhi[d];
+        }
+        int fl[3], fnl[3];
+        const int nf = node_faces<Nq>(i, j, k, fl, fnl);
+        for (int c = 0; c < nf; ++c) {
+            const int f = fl[c], fn = fnl[c];
+            const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
 #pragma unroll
-            for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
-            Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
-            refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
+            for (int s = 0; s < 5; ++s) T[s] -= w * sFl[f * Nfp + fn][s];
+        }
+        double *tb = tend + (int64_t)Np * 5 * e + n;
+        if (MODE == 0) {
+#pragma unroll
+            for (int s = 0; s < 5; ++s)
+                tb[Np * s] = (beta != 0.0) ? alpha * (double)T[s] + beta * tb[Np * s] : alpha * (double)T[s];
         } else {
+            double *qo = Qout + (int64_t)Np * 5 * e + n;
 #pragma unroll
-            for (int s = 0; s < 5; ++s) qp[s] = q[s];
-            qp[1 + d] = -q[1 + d];  // reflect the normal momentum
+            for (int s = 0; s < 5; ++s) {
+                const double dd = (double)T[s] + tb[Np * s];
+                qn[s] = q[s] + bdt * dd;
+                qo[Np * s] = qn[s];
+                tb[Np * s] = dd * rka;
+            }
         }
-        const ThermoD tp = thermo_d(P, qp, Php);
-        float qpf[5], upf[3], jump[5];
-#pragma unroll
-        for (int s = 0; s < 5; ++s) {
-            qpf[s] = (float)qp[s];
-            jump[s] = (float)(q[s] - qp[s]);
-        }
-        const float rip = 1.0f / qpf[0];
-        upf[0] = qpf[1] * rip;
-        upf[1] = qpf[2] * rip;
-        upf[2] = qpf[3] * rip;
-        float Fm[5], Fp[5], flux[5];
-        flux1_row_f(qf, uf, pf, pmf, d, Fm);
-        flux1_row_f(qpf, upf, (float)tp.p, (float)(MP.ref_sub ? tp.p - refpp : tp.p - p_e), d, Fp);
-#pragma unroll
-        for (int s = 0; s < 5; ++s) flux[s] = (Fm[s] + Fp[s]) * h;
-        if (MP.rusanov) {
-            const float umd = d == 0 ? uf[0] : (d == 1 ? uf[1] : uf[2]);
-            const float upd = d == 0 ? upf[0] : (d == 1 ? upf[1] : upf[2]);
-            const float mw = 0.5f * fmaxf(fabsf(umd) + cm, fabsf(upd) + sqrtf(MP.gammaR * (float)tp.temp));
-#pragma unroll
-            for (int s = 0; s < 5; ++s) flux[s] += mw * jump[s];
-        }
-        if (DIFF && !wall) {
-            flux2_row_f<SMAG>(MP, qf, gf, nu, d, Fm);
-#pragma unroll
-            for (int v = 0; v < TR_F2; ++v) flux[1 + v] += (Fm[1 + v] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
-        }
-#pragma unroll
-        for (int s = 0; s < 5; ++s) T[s] -= w * flux[s];
     }
-
-    double *tb = tend + (int64_t)Np * 5 * e + n;
-    if (MODE == 0) {
+    if (MODE == 1 && qtr_out) {
+        // the next stage's state trace, written face-major (contiguous per face)
+        double(*sQ)[Np] = reinterpret_cast<double(*)[Np]>(&sF[0][0][0]);
+        __syncthreads();
+        if (act)
 #pragma unroll
-        for (int s = 0; s < 5; ++s) tb[Np * s] = (beta != 0.0) ? alpha * (double)T[s] + beta * tb[Np * s] : alpha * (double)T[s];
-    } else {
-        double *qo = Qout + (int64_t)Np * 5 * e + n;
-        double qn[5];
+            for (int s = 0; s < 5; ++s) sQ[s][n] = qn[s];
+        __syncthreads();
+        for (int t = n; t < 6 * Nfp; t += NT) {
+            const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
 #pragma unroll
-        for (int s = 0; s < 5; ++s) {
-            const double dd = (double)T[s] + tb[Np * s];
-            qn[s] = q[s] + bdt * dd;
-            qo[Np * s] = qn[s];
-            tb[Np * s] = dd * rka;
+            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sQ[s][vn];
         }
-        if (qtr_out)
-            for (int c = 0; c < nf; ++c)
-#pragma unroll
-                for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, fl[c], s, TR_Q, fnl[c])] = qn[s];
     }
 }
 
