@@ -3,7 +3,7 @@
 The closure replaces nu_0 of SmagorinskyLilly
 (src/Common/TurbulenceClosures/TurbulenceClosures.jl:478-491) by
 
-    nu_0 = exp(min(y, 8)) * Delta^2 * |S| + 1e-5,   y = MLP((raw - shift) * scale),
+    nu_0 = exp(min(y, -1)) * Delta^2 * |S| + 1e-5,   y = MLP((raw - shift) * scale),
 
 raw = (du_c/dx_d at 3c + d, Delta), a 10 -> 32 -> 32 -> 1 network with SiLU
 hidden units.  The right-hand side evaluates it with BF16 MFMA tiles fused into
@@ -30,7 +30,7 @@ NPARAM = abi.MLP_NPARAM
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3 = 0, 320, 352, 1376, 1408, 1440
 NIN, NH = 10, 32
 SAMPLE_W = 13        # raw inputs (10), then P, Q, R of the stress loss
-YMAX = 8.0
+YMAX = -1.0          # output clamp: nu_0 <= 0.37 Delta^2 |S| (C_s <= 0.6)
 NMOM = 11            # filtered fields: rho, rho u (3), rho e, rho u_a u_b (6)
 
 
@@ -204,12 +204,14 @@ def box_filter(Qf: torch.Tensor, fine: BrickGrid, coarse: BrickGrid, out: Option
 
 
 def closure_samples(M: torch.Tensor, Gu: torch.Tensor, grid: BrickGrid, Delta: float, nu_mol: float,
-                    table: torch.Tensor, offset: int):
-    """Append the grid's samples to the SoA table (13, ld) at column `offset`."""
+                    table: torch.Tensor, offset: int, element_average: bool = False):
+    """Append the grid's samples to the SoA table (13, ld) at column `offset`;
+    element_average replaces P, Q, R by their element means."""
     lib = abi.load_library()
     ld = table.shape[1]
-    _check(lib.cmdg_closure_samples(_vp(M), _vp(Gu), grid.nelem, grid.N, float(Delta), float(nu_mol), _vp(table),
-                                    ld, int(offset), _stream(M.device)), "cmdg_closure_samples")
+    _check(lib.cmdg_closure_samples(_vp(M), _vp(Gu), grid.nelem, grid.N, float(Delta), float(nu_mol),
+                                    int(bool(element_average)), _vp(table), ld, int(offset), _stream(M.device)),
+           "cmdg_closure_samples")
     return offset + grid.nelem * grid.Np
 
 

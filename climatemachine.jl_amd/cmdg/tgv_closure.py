@@ -10,7 +10,8 @@ kernels and saved.
 Stage ``eval``: a plain fine run of the unshifted vortex gives the reference
 (its filtered state); the coarse brick is run from the filtered initial state with
 (a) the learned closure (mixed precision), (b) Smagorinsky-Lilly (mixed and FP64)
-and (c) no closure (FP64, the molecular viscosity only); kinetic-energy spectra
+and (c) no closure (FP64, the molecular viscosity only) and (d) optionally the constant network
+of the coefficient fitted to the training data (--fitted-cs); kinetic-energy spectra
 (power_spectrum_3d on p points per element) and total kinetic energy are compared
 with the filtered reference at the evaluation times.
 
@@ -54,7 +55,7 @@ def model(prm, grid, kind: str, nu: float, mlp=None, C_smag: float = 0.21):
     if kind in ("smag", "smag64"):
         cfg = atmos.AtmosConfig(prm, turbulence=abi.SMAGORINSKY_LILLY, C_smag=C_smag)
         return DGModel(cfg, grid, precision=abi.MIXED if kind == "smag" else abi.FP64)
-    if kind == "learned":
+    if kind in ("learned", "fitted"):
         cfg = atmos.AtmosConfig(prm, turbulence=abi.LEARNED_CLOSURE, C_smag=C_smag)
         dg = DGModel(cfg, grid, precision=abi.MIXED)
         dg.set_closure(mlp)
@@ -124,7 +125,8 @@ def stage_train(a):
     def snap(step, t):
         M = closure.box_filter(Q, fine, coarse)
         Gu = dgc.velocity_gradient(M[:, :5].contiguous())
-        state["off"] = closure.closure_samples(M, Gu, coarse, Delta, nu, table, state["off"])
+        state["off"] = closure.closure_samples(M, Gu, coarse, Delta, nu, table, state["off"],
+                                               element_average=bool(a.average))
 
     log(f"[train] dt {dt:.4e} s, {nsteps} steps to t* = {a.tend}, snapshots every {every} steps, {ld} samples")
     el = run(dg, Q, dt, nsteps, snap, every, "fine")
@@ -200,7 +202,10 @@ def stage_eval(a):
     out = dict(stage="eval", fine=a.fine, coarse=a.coarse, U0=U0, nu=nu, tend=a.tend, dt_fine=dt_f,
                steps_fine=nf, steps_coarse=nc, fine_seconds=el_f, points_per_element=p, models={})
     for kind in a.models.split(","):
-        dgc = model(prm, coarse, kind, nu, mlp)
+        if kind == "fitted" and a.fitted_cs <= 0:
+            continue
+        net = closure.MLPParams.constant(2 * math.log(a.fitted_cs)) if kind == "fitted" else mlp
+        dgc = model(prm, coarse, kind, nu, net)
         Qc = Qc0.clone()
         res = {}
 
@@ -244,11 +249,14 @@ def main(argv=None):
     ap.add_argument("--snapshots", type=int, default=24)
     ap.add_argument("--phase", type=int, default=1, help="seeded phase shift of the training vortex")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--train-steps", type=int, default=4000)
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--average", type=int, default=1, help="element-averaged P, Q, R targets")
+    ap.add_argument("--fitted-cs", type=float, default=0.0,
+                    help="also evaluate the constant network of this coefficient (model 'fitted')")
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--weights", default=None)
-    ap.add_argument("--models", default="learned,smag,smag64,none")
+    ap.add_argument("--models", default="learned,fitted,smag,smag64,none")
     ap.add_argument("--eval-times", default="4,6,8")
     ap.add_argument("--points", type=int, default=4)
     ap.add_argument("--out", required=True)

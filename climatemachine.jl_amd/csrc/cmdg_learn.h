@@ -191,6 +191,26 @@ __global__ void k_closure_samples(const double *__restrict__ M, const double *__
     o[ld * 12] = (float)R;
 }
 
+// Replaces P, Q, R of each sample by their mean over the sample's element (the local
+// average of the dynamic procedure): the optimum coefficient of a node is then the
+// element's least-squares coefficient -sum Q / sum P, not the noisy pointwise one.
+__global__ void __launch_bounds__(128) k_element_average_pqr(float *__restrict__ out, int64_t ld, int64_t offset,
+                                                             int Np) {
+    __shared__ double red[3][128];
+    const int64_t e = blockIdx.x;
+    const int n = threadIdx.x;
+    float *o = out + offset + e * Np;
+    for (int r = 0; r < 3; ++r) red[r][n] = n < Np ? (double)o[ld * (10 + r) + n] : 0.0;
+    __syncthreads();
+    for (int w = 64; w > 0; w >>= 1) {
+        if (n < w)
+            for (int r = 0; r < 3; ++r) red[r][n] += red[r][n + w];
+        __syncthreads();
+    }
+    if (n < Np)
+        for (int r = 0; r < 3; ++r) o[ld * (10 + r) + n] = (float)(red[r][0] / Np);
+}
+
 // ------------------------------------------------------------------ training
 // FP32 master parameters, flat: W1 [32][10] at 0, b1 at 320, W2 [32][32] at 352,
 // b2 at 1376, W3 [32] at 1408, b3 at 1440.

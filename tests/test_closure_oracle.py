@@ -36,7 +36,7 @@ def _samples(rng, n):
 
 def test_loss_gradient_finite_differences():
     rng = np.random.default_rng(1)
-    p = closure.MLPParams.random(seed=3, y0=-1.0).flat.astype(np.float64)
+    p = closure.MLPParams.random(seed=3, y0=-4.0).flat.astype(np.float64)
     x, P, Q, R = _samples(rng, 64)
     L, g = MO.loss_grad(p, x, P, Q, R, 0.5)
     assert abs(L - MO.loss(p, x, P, Q, R, 0.5)) < 1e-12 * abs(L)
@@ -53,7 +53,7 @@ def test_loss_gradient_finite_differences():
 
 def test_loss_gradient_clamped_output_is_flat():
     rng = np.random.default_rng(2)
-    p = closure.MLPParams.constant(9.0).flat.astype(np.float64)   # y = 9 > 8: clamped
+    p = closure.MLPParams.constant(0.0).flat.astype(np.float64)   # y = 0 > -1: clamped
     x, P, Q, R = _samples(rng, 8)
     _, g = MO.loss_grad(p, x, P, Q, R)
     assert np.all(g == 0.0)

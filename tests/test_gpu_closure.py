@@ -80,7 +80,7 @@ def test_learned_rhs_matches_oracle():
     prm, grid = _tgv_case(3)
     Q0 = _tgv_state(prm, grid)
     # inputs of order one: velocity gradients of this flow are O(100) 1/s
-    mlp = _net(seed=7, y0=-3.0, scale=np.r_[np.full(9, 0.01), 10.0])
+    mlp = _net(seed=7, y0=-4.0, scale=np.r_[np.full(9, 0.01), 10.0])
     cfg, aux, dg = _learned_model(grid, prm, mlp)
     orc = O.Oracle(cfg, grid, aux)
     orc.set_mlp(mlp)
@@ -134,7 +134,7 @@ def test_training_gradient_and_adam_match_oracle():
     P = rng.uniform(0.1, 2.0, n)
     Qv = -rng.uniform(0, 1, n) * P
     table[10, :n], table[11, :n], table[12, :n] = P, Qv, Qv * Qv / P + rng.uniform(0, 0.5, n)
-    mlp = _net(seed=9, y0=-1.0, shift=rng.normal(0, 0.1, 10), scale=rng.uniform(0.3, 1.0, 10))
+    mlp = _net(seed=9, y0=-4.0, shift=rng.normal(0, 0.1, 10), scale=rng.uniform(0.3, 1.0, 10))
     tr = closure.ClosureTrainer(mlp, max_batch=4096)
     T = torch.from_numpy(table).cuda()
     idx = torch.from_numpy(rng.permutation(n)[:700].astype(np.int64)).cuda()
@@ -300,6 +300,15 @@ def test_box_filter_and_samples():
         assert rel_l2(T[row], val) < 1e-6, row
     assert np.allclose(T[:9], G.reshape(9, n), rtol=1e-6, atol=1e-6 * np.abs(G).max())
     assert np.all(T[9] == np.float32(Delta))
+    # element-averaged targets: P, Q, R replaced by their element means
+    table2 = torch.zeros((13, n), dtype=torch.float32, device="cuda")
+    closure.closure_samples(M, Gu, coarse, Delta, nu, table2, 0, element_average=True)
+    T2 = table2.cpu().numpy()
+    assert np.array_equal(T2[:10], T[:10])
+    for row in (10, 11, 12):
+        mean = T[row].astype(np.float64).reshape(coarse.nelem, coarse.Np).mean(1)
+        assert np.allclose(T2[row].reshape(coarse.nelem, coarse.Np), mean[:, None], rtol=1e-6,
+                           atol=1e-6 * np.abs(mean).max()), row
     dg.close()
 
 
