@@ -131,21 +131,6 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
     G[4] = SMAG ? (float)t.temp / powf((float)(t.p / P.MSLP), M.kappa) : 0.0f;
 }
 
-// State across face node fn of face f of element e for the gradient flux: the
-// neighbour's state trace and Phi, or on a wall the element's own (the wall state is
-// formed by the caller).
-template <int Nfp>
-__device__ __forceinline__ bool face_neighbour_state(const AffArrays &A, const double *__restrict__ qtr, int64_t e,
-                                                     int f, int fn, double qp[5], double &Php) {
-    const int32_t ne = A.nbe[6 * e + f];
-    const int64_t eo = ne >= 0 ? ne : e;
-    const int fo = ne >= 0 ? (f ^ 1) : f;
-#pragma unroll
-    for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(eo, fo, s, TR_Q, fn)];
-    Php = A.str[tri<Nfp>(eo, fo, 0, TR_S, fn)];
-    return ne < 0;
-}
-
 // ------------------------------------------------------------------ gradient (mixed)
 template <int Nq, bool SMAG, bool LEARN>
 __global__ void __launch_bounds__(threads_for<Nq>())
@@ -165,11 +150,6 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     const int n = threadIdx.x;
-    // the first face task's neighbour state is loaded before anything else: it does
-    // not depend on the element's own work, so its latency overlaps with it
-    static_assert(threads_for<Nq>() <= 6 * Nq * Nq, "every lane has a first face task");
-    double qp0[5], Php0;
-    const bool wall0 = face_neighbour_state<Nfp>(A, qtr, e, n / Nfp, n % Nfp, qp0, Php0);
     if (n < Nq * Nq) sD[n] = (float)A.D[n];
     const bool act = n < Np;
     const int i = n % Nq, j = (n / Nq) % Nq, k = n / (Nq * Nq);
@@ -204,25 +184,29 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     for (int t = n; t < 6 * Nfp; t += NT) {
         const int f = t / Nfp, fn = t % Nfp, d = f >> 1, vn = face_node<Nq>(f, fn);
         const float nd = (f & 1) ? 1.0f : -1.0f;
-        double qp[5], Php;
-        bool wall;
-        if (t == n) {
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qp[s] = qp0[s];
-            Php = Php0;
-            wall = wall0;
-        } else {
-            wall = face_neighbour_state<Nfp>(A, qtr, e, f, fn, qp, Php);
-        }
+        const int32_t ne = A.nbe[6 * e + f];
         float Gm[5], Gp[5];
 #pragma unroll
         for (int s = 0; s < 5; ++s) Gm[s] = s < NG ? sG[s][vn] : 0.0f;
         float jump[5];
-        if (wall) qp[1 + d] = 0.0;  // the gradient wall state removes the normal momentum
-        grad_arg_f<SMAG>(P, MP, qp, thermo_d(P, qp, Php), Gp);
-        const float hj = wall ? 1.0f : 0.5f;
+        if (ne >= 0) {
+            const int fo = f ^ 1;
+            double qp[5];
 #pragma unroll
-        for (int s = 0; s < 5; ++s) jump[s] = nd * (hj * (Gp[s] - Gm[s]));
+            for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
+            const double Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+            grad_arg_f<SMAG>(P, MP, qp, thermo_d(P, qp, Php), Gp);
+#pragma unroll
+            for (int s = 0; s < 5; ++s) jump[s] = nd * (0.5f * (Gp[s] - Gm[s]));
+        } else {
+            double qb[5];
+#pragma unroll
+            for (int s = 0; s < 5; ++s) qb[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
+            qb[1 + d] = 0.0;  // the gradient wall state removes the normal momentum
+            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, A.str[tri<Nfp>(e, f, 0, TR_S, fn)]), Gp);
+#pragma unroll
+            for (int s = 0; s < 5; ++s) jump[s] = nd * (Gp[s] - Gm[s]);
+        }
 #pragma unroll
         for (int s = 0; s < 5; ++s) sJ[t][s] = jump[s];
     }
@@ -298,59 +282,41 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 }
 
 // ------------------------------------------------------------------ tendency (mixed)
-// Both sides of face node fn of face f of element e, read from the face traces.
-template <bool DIFF> struct FaceDataMix {
-    double q[5], qp[5], Phi, ref_p, Php, refpp;
-    float f2m[DIFF ? TR_F2 : 1], f2p[DIFF ? TR_F2 : 1];
-    bool wall;
-};
-
+// Numerical flux at face node fn of face f of element e (FP32 fluxes from the FP64
+// face traces; the jump q- - q+ and the pressure perturbation in FP64).
 template <int Nq, bool DIFF>
-__device__ __forceinline__ void load_face_mix(const AffArrays &A, const double *__restrict__ qtr,
-                                              const float *__restrict__ f2tr, int64_t e, int f, int fn,
-                                              FaceDataMix<DIFF> &D) {
-    constexpr int Nfp = Nq * Nq;
-    const int32_t ne = A.nbe[6 * e + f];
-    D.wall = ne < 0;
-    const int64_t eo = D.wall ? e : ne;
-    const int fo = D.wall ? f : (f ^ 1);
-#pragma unroll
-    for (int s = 0; s < 5; ++s) {
-        D.q[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
-        D.qp[s] = qtr[tri<Nfp>(eo, fo, s, TR_Q, fn)];
-    }
-    D.Phi = A.str[tri<Nfp>(e, f, 0, TR_S, fn)];
-    D.ref_p = A.str[tri<Nfp>(e, f, 1, TR_S, fn)];
-    D.Php = A.str[tri<Nfp>(eo, fo, 0, TR_S, fn)];
-    D.refpp = A.str[tri<Nfp>(eo, fo, 1, TR_S, fn)];
-    if (DIFF)
-#pragma unroll
-        for (int v = 0; v < TR_F2; ++v) {
-            D.f2m[v] = f2tr[tri<Nfp>(e, f, v, TR_F2, fn)];
-            D.f2p[v] = f2tr[tri<Nfp>(eo, fo, v, TR_F2, fn)];
-        }
-}
-
-// Numerical flux (FP32 fluxes from the FP64 face states; the jump q- - q+ and the
-// pressure perturbation in FP64) at a face node of face f, seen from its element.
-template <bool DIFF>
-__device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, int f, FaceDataMix<DIFF> &D,
-                                              double p_e, float flux[5]) {
+__device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
+                                              int f, int fn, const double *__restrict__ qtr,
+                                              const float *__restrict__ f2tr, double p_e, float flux[5]) {
 #pragma clang fp contract(fast)
+    constexpr int Nfp = Nq * Nq;
     const int d = f >> 1;
     const float h = (f & 1) ? 0.5f : -0.5f;
-    if (D.wall) {
+    const int32_t ne = A.nbe[6 * e + f];
+    const bool wall = ne < 0;
+    const int fo = f ^ 1;
+    double q[5], qp[5];
 #pragma unroll
-        for (int s = 0; s < 5; ++s) D.qp[s] = D.q[s];
-        D.qp[1 + d] = -D.q[1 + d];  // reflect the normal momentum
+    for (int s = 0; s < 5; ++s) q[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
+    const double Phi = A.str[tri<Nfp>(e, f, 0, TR_S, fn)], ref_p = A.str[tri<Nfp>(e, f, 1, TR_S, fn)];
+    double Php = Phi, refpp = ref_p;
+    if (!wall) {
+#pragma unroll
+        for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
+        Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+        refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
+    } else {
+#pragma unroll
+        for (int s = 0; s < 5; ++s) qp[s] = q[s];
+        qp[1 + d] = -q[1 + d];  // reflect the normal momentum
     }
-    const ThermoD tm = thermo_d(P, D.q, D.Phi), tp = thermo_d(P, D.qp, D.Php);
+    const ThermoD tm = thermo_d(P, q, Phi), tp = thermo_d(P, qp, Php);
     float qf[5], qpf[5], uf[3], upf[3], jump[5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
-        qf[s] = (float)D.q[s];
-        qpf[s] = (float)D.qp[s];
-        jump[s] = (float)(D.q[s] - D.qp[s]);
+        qf[s] = (float)q[s];
+        qpf[s] = (float)qp[s];
+        jump[s] = (float)(q[s] - qp[s]);
     }
     const float rim = 1.0f / qf[0], rip = 1.0f / qpf[0];
 #pragma unroll
@@ -359,8 +325,8 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
         upf[c] = qpf[1 + c] * rip;
     }
     float Fm[5], Fp[5];
-    flux1_row_f(qf, uf, (float)tm.p, (float)(MP.ref_sub ? tm.p - D.ref_p : tm.p - p_e), d, Fm);
-    flux1_row_f(qpf, upf, (float)tp.p, (float)(MP.ref_sub ? tp.p - D.refpp : tp.p - p_e), d, Fp);
+    flux1_row_f(qf, uf, (float)tm.p, (float)(MP.ref_sub ? tm.p - ref_p : tm.p - p_e), d, Fm);
+    flux1_row_f(qpf, upf, (float)tp.p, (float)(MP.ref_sub ? tp.p - refpp : tp.p - p_e), d, Fp);
 #pragma unroll
     for (int s = 0; s < 5; ++s) flux[s] = (Fm[s] + Fp[s]) * h;
     if (MP.rusanov) {
@@ -371,9 +337,10 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
 #pragma unroll
         for (int s = 0; s < 5; ++s) flux[s] += mw * jump[s];
     }
-    if (DIFF && !D.wall) {
+    if (DIFF && !wall) {
 #pragma unroll
-        for (int v = 0; v < TR_F2; ++v) flux[1 + v] += (D.f2m[v] + D.f2p[v]) * h;
+        for (int v = 0; v < TR_F2; ++v)
+            flux[1 + v] += (f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
     }
 }
 
@@ -393,11 +360,6 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     const int n = threadIdx.x;
-    // the first face task's data is loaded before anything else: it does not depend
-    // on the element's own work, so its latency overlaps with it
-    static_assert(NT <= 6 * Nfp, "every lane has a first face task");
-    FaceDataMix<DIFF> fd0;
-    load_face_mix<Nq, DIFF>(A, qtr, f2tr, e, n / Nfp, n % Nfp, fd0);
     if (n < Nq * Nq) sD[n] = (float)A.D[n];
     const bool act = n < Np;
     const int i = n % Nq, j = (n / Nq) % Nq, k = n / (Nq * Nq);
@@ -474,13 +436,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         float fl5[5];
-        if (t == n) {
-            face_flux_mix<DIFF>(P, MP, t / Nfp, fd0, p_e, fl5);
-        } else {
-            FaceDataMix<DIFF> fd;
-            load_face_mix<Nq, DIFF>(A, qtr, f2tr, e, t / Nfp, t % Nfp, fd);
-            face_flux_mix<DIFF>(P, MP, t / Nfp, fd, p_e, fl5);
-        }
+        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, qtr, f2tr, p_e, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }
