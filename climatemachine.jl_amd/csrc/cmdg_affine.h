@@ -105,8 +105,9 @@ k_trace_aff(int64_t nelem, const double *__restrict__ Q, double *__restrict__ qt
 // element's own gradient arguments in LDS, q+ from the neighbour's state trace.
 template <int Nq, bool SMAG>
 __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
-                                                  const double *__restrict__ qtr, const double *__restrict__ aux,
-                                                  const double (*sG)[Nq * Nq * Nq], double *dgf) {
+                                                  int32_t ne, const double *__restrict__ qtr,
+                                                  const double *__restrict__ aux, const double (*sG)[Nq * Nq * Nq],
+                                                  double *dgf) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
     const double nd = (f & 1) ? 1.0 : -1.0;
@@ -120,7 +121,6 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
 #pragma unroll
         for (int c = 0; c < 3; ++c) gPhi[c] = ab[(int64_t)Np * (P.a_gPhi + c)];
     }
-    const int32_t ne = A.nbe[6 * e + f];
     double Gp[5], Gs[5], nG[5];
     if (ne >= 0) {
         const int fo = f ^ 1;
@@ -167,6 +167,9 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     double(*sJ)[NGF] = reinterpret_cast<double(*)[NGF]>(sJbuf);
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
+    int32_t nb[6];  // the element's face neighbours (uniform: scalar loads, issued first)
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nb[f] = A.nbe[6 * e + f];
     const int n = threadIdx.x;
     if (n < Nq * Nq) sD[n] = A.D[n];
     const bool act = n < Np;
@@ -194,7 +197,7 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     // lifted central jumps of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double dgf[NGF];
-        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, qtr, aux, sG, dgf);
+        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], qtr, aux, sG, dgf);
 #pragma unroll
         for (int s = 0; s < NGF; ++s) sJ[t][s] = dgf[s];
     }
@@ -270,13 +273,12 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
 // walls bc_momentum.jl:24-49 / bc_energy.jl:10-17 (zero second-order flux).
 template <int Nq, bool DIFF, bool SMAG>
 __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
-                                              const double *__restrict__ qtr, const double *__restrict__ f2tr,
-                                              double flux[5]) {
+                                              int32_t ne, const double *__restrict__ qtr,
+                                              const double *__restrict__ f2tr, double flux[5]) {
     constexpr int Nfp = Nq * Nq;
     const int d = f >> 1;
     const double nd = (f & 1) ? 1.0 : -1.0;
     const double h = nd * 0.5;
-    const int32_t ne = A.nbe[6 * e + f];
     const bool wall = ne < 0;
     const int fo = f ^ 1;
     double q[5], qp[5];
@@ -341,6 +343,9 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     __shared__ double sFl[6 * Nfp][5];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
+    int32_t nb[6];  // the element's face neighbours (uniform: scalar loads, issued first)
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nb[f] = A.nbe[6 * e + f];
     const int n = threadIdx.x;
     if (n < Nq * Nq) sD[n] = A.D[n];
     const bool act = n < Np;
@@ -392,7 +397,7 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double fl5[5];
-        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, qtr, f2tr, fl5);
+        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], qtr, f2tr, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }

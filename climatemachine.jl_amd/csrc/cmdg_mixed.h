@@ -149,6 +149,9 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     __shared__ uint16_t sX[LEARN ? (NT / 64) * MLP_LDS_PER_WAVE : 1];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
+    int32_t nb[6];  // the element's face neighbours (uniform: scalar loads, issued first)
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nb[f] = A.nbe[6 * e + f];
     const int n = threadIdx.x;
     if (n < Nq * Nq) sD[n] = (float)A.D[n];
     const bool act = n < Np;
@@ -184,7 +187,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     for (int t = n; t < 6 * Nfp; t += NT) {
         const int f = t / Nfp, fn = t % Nfp, d = f >> 1, vn = face_node<Nq>(f, fn);
         const float nd = (f & 1) ? 1.0f : -1.0f;
-        const int32_t ne = A.nbe[6 * e + f];
+        const int32_t ne = nb[f];
         float Gm[5], Gp[5];
 #pragma unroll
         for (int s = 0; s < 5; ++s) Gm[s] = s < NG ? sG[s][vn] : 0.0f;
@@ -286,13 +289,12 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 // face traces; the jump q- - q+ and the pressure perturbation in FP64).
 template <int Nq, bool DIFF>
 __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
-                                              int f, int fn, const double *__restrict__ qtr,
+                                              int f, int fn, int32_t ne, const double *__restrict__ qtr,
                                               const float *__restrict__ f2tr, double p_e, float flux[5]) {
 #pragma clang fp contract(fast)
     constexpr int Nfp = Nq * Nq;
     const int d = f >> 1;
     const float h = (f & 1) ? 0.5f : -0.5f;
-    const int32_t ne = A.nbe[6 * e + f];
     const bool wall = ne < 0;
     const int fo = f ^ 1;
     double q[5], qp[5];
@@ -359,6 +361,9 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     __shared__ float sFl[6 * Nfp][5];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
+    int32_t nb[6];  // the element's face neighbours (uniform: scalar loads, issued first)
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nb[f] = A.nbe[6 * e + f];
     const int n = threadIdx.x;
     if (n < Nq * Nq) sD[n] = (float)A.D[n];
     const bool act = n < Np;
@@ -436,7 +441,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         float fl5[5];
-        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, qtr, f2tr, p_e, fl5);
+        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], qtr, f2tr, p_e, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }
