@@ -105,8 +105,8 @@ def load_library(path: str = None):
     lib.cmdg_mlp_forward.argtypes = [vp, vp, vp, C.c_int64]
     lib.cmdg_velocity_gradient.argtypes = [vp, vp, vp]
     lib.cmdg_box_filter.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp, vp]
-    lib.cmdg_closure_samples.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_int32, vp,
-                                         C.c_int64, C.c_int64, vp]
+    lib.cmdg_closure_samples.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                         C.c_int32, vp, C.c_int64, C.c_int64, vp]
     lib.cmdg_mlp_loss_grad.argtypes = [vp, vp, vp, vp, C.c_int64, vp, C.c_int64, C.c_double, vp, vp, C.c_int64, vp]
     lib.cmdg_mlp_adam.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int64,
                                   vp]

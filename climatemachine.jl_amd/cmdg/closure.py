@@ -3,10 +3,10 @@
 The closure replaces nu_0 of SmagorinskyLilly
 (src/Common/TurbulenceClosures/TurbulenceClosures.jl:478-491) by
 
-    nu_0 = exp(min(y, -1)) * Delta^2 * |S| + 1e-5,   y = MLP((raw - shift) * scale),
+    nu_0 = exp(min(y, -1)) * Delta^2 * |S| + nu_floor,   y = MLP((raw - shift) * scale),
 
 raw = (du_c/dx_d at 3c + d, Delta), a 10 -> 32 -> 32 -> 1 network with SiLU
-hidden units.  The right-hand side evaluates it with BF16 MFMA tiles fused into
+hidden units, nu_floor the context's kinematic viscosity (1e-5 when zero).  The right-hand side evaluates it with BF16 MFMA tiles fused into
 the mixed-precision gradient kernel; this module holds the FP32 master
 parameters, exports them as BF16, and drives the library's training, filter,
 sample and spectrum kernels.  It has no counterpart in the reference except the
@@ -204,14 +204,15 @@ def box_filter(Qf: torch.Tensor, fine: BrickGrid, coarse: BrickGrid, out: Option
 
 
 def closure_samples(M: torch.Tensor, Gu: torch.Tensor, grid: BrickGrid, Delta: float, nu_mol: float,
-                    table: torch.Tensor, offset: int, element_average: bool = False):
+                    table: torch.Tensor, offset: int, element_average: bool = False, nu_floor: float = 1e-5):
     """Append the grid's samples to the SoA table (13, ld) at column `offset`;
-    element_average replaces P, Q, R by their element means."""
+    nu_floor is the closure's viscosity floor (its context's `viscosity`, 1e-5 when
+    zero); element_average replaces P, Q, R by their element means."""
     lib = abi.load_library()
     ld = table.shape[1]
     _check(lib.cmdg_closure_samples(_vp(M), _vp(Gu), grid.nelem, grid.N, float(Delta), float(nu_mol),
-                                    int(bool(element_average)), _vp(table), ld, int(offset), _stream(M.device)),
-           "cmdg_closure_samples")
+                                    float(nu_floor), int(bool(element_average)), _vp(table), ld, int(offset),
+                                    _stream(M.device)), "cmdg_closure_samples")
     return offset + grid.nelem * grid.Np
 
 

@@ -47,7 +47,9 @@ def setup(n: int, mach: float, re: float, L: float = 1.0):
 
 
 def model(prm, grid, kind: str, nu: float, mlp=None, C_smag: float = 0.21):
-    """kind: fine / none (constant kinematic viscosity), smag64, smag, learned."""
+    """kind: fine / none (constant kinematic viscosity nu), smag / smag64 (reference
+    Smagorinsky-Lilly, mixed / FP64), net (the learned closure, mixed precision, with
+    nu as its viscosity floor, running network `mlp`)."""
     if kind in ("fine", "none"):
         cfg = atmos.AtmosConfig(prm, turbulence=abi.CONSTANT_KINEMATIC_VISCOSITY, viscosity=nu,
                                 with_divergence=True)
@@ -55,8 +57,8 @@ def model(prm, grid, kind: str, nu: float, mlp=None, C_smag: float = 0.21):
     if kind in ("smag", "smag64"):
         cfg = atmos.AtmosConfig(prm, turbulence=abi.SMAGORINSKY_LILLY, C_smag=C_smag)
         return DGModel(cfg, grid, precision=abi.MIXED if kind == "smag" else abi.FP64)
-    if kind in ("learned", "fitted"):
-        cfg = atmos.AtmosConfig(prm, turbulence=abi.LEARNED_CLOSURE, C_smag=C_smag)
+    if kind == "net":
+        cfg = atmos.AtmosConfig(prm, turbulence=abi.LEARNED_CLOSURE, C_smag=C_smag, viscosity=nu)
         dg = DGModel(cfg, grid, precision=abi.MIXED)
         dg.set_closure(mlp)
         return dg
@@ -118,59 +120,77 @@ def stage_train(a):
     nper = coarse.nelem * coarse.Np
     nsnap = nsteps // every + (1 if nsteps % every else 0)
     ld = nsnap * nper
-    table = torch.zeros((closure.SAMPLE_W, ld), dtype=torch.float32, device="cuda")
+    # two tables from the same run: pointwise and element-averaged targets; the
+    # closure's floor is the molecular viscosity, so the network supplies the
+    # subgrid stress alone
+    tables = {v: torch.zeros((closure.SAMPLE_W, ld), dtype=torch.float32, device="cuda")
+              for v in ("pointwise", "average")}
     Delta = float(atmos.lengthscale(coarse).ravel()[0])
     state = {"off": 0}
 
     def snap(step, t):
         M = closure.box_filter(Q, fine, coarse)
         Gu = dgc.velocity_gradient(M[:, :5].contiguous())
-        state["off"] = closure.closure_samples(M, Gu, coarse, Delta, nu, table, state["off"],
-                                               element_average=bool(a.average))
+        off = state["off"]
+        for v, tab in tables.items():
+            state["off"] = closure.closure_samples(M, Gu, coarse, Delta, nu, tab, off,
+                                                   element_average=(v == "average"), nu_floor=nu)
 
     log(f"[train] dt {dt:.4e} s, {nsteps} steps to t* = {a.tend}, snapshots every {every} steps, {ld} samples")
     el = run(dg, Q, dt, nsteps, snap, every, "fine")
+    dg.close()
     n = state["off"]
-    tab = table[:, :n]
-    g = tab[:9]
-    rms = float(torch.sqrt((g.double() ** 2).mean()).item())
-    mlp0 = closure.MLPParams.random(seed=a.seed, y0=2 * math.log(0.17),
-                                    shift=np.zeros(10), scale=np.r_[np.full(9, 1.0 / rms), 1.0 / Delta])
-    P, Qv, R = (tab[10].double(), tab[11].double(), tab[12].double())
-    # the pointwise optimum exp(y*) = -Q/P where the target is dissipative
-    opt = (-Qv / P.clamp_min(1e-30)).clamp_min(0)
-    c2 = float((-(Qv.sum()) / P.sum()).item())
-    log(f"[train] {n} samples; grad-u rms {rms:.3e} 1/s; Delta {Delta:.4e} m; best constant exp(y) {c2:.4e} "
-        f"(C_s {math.sqrt(max(c2, 0)):.3f}); pointwise optimum median {float(opt.median()):.3e}")
-    tr = closure.ClosureTrainer(mlp0, max_batch=a.batch)
-    inv_norm = 1.0 / float(R.mean().item())
-    t0 = time.time()
-    hist = tr.fit(table, n, steps=a.train_steps, batch=a.batch, lr=a.lr, lr_final=a.lr / 30, inv_norm=inv_norm,
-                  seed=a.seed, log=log, log_every=max(1, a.train_steps // 20))
-    torch.cuda.synchronize()
-    t_train = time.time() - t0
-    # loss of the best constant coefficient (Smagorinsky with the optimal C_s) and of C_s = 0.21
-    def const_loss(e):
-        return float(((e * e * P + 2 * e * Qv + R).mean() / R.mean()).item())
-    mlp = tr.mlp()
-    full = tr.loss_grad(table, min(n, a.batch), None, inv_norm / min(n, a.batch))
     res = dict(stage="train", fine=a.fine, coarse=a.coarse, mach=a.mach, re=a.re, U0=U0, nu=nu, phase=phase,
-               dt=dt, steps=nsteps, fine_seconds=el, samples=n, grad_rms=rms, Delta=Delta,
-               train_steps=a.train_steps, batch=a.batch, train_seconds=t_train, history=hist,
-               loss_first_batch_trained=float(full[closure.NPARAM].item()),
-               loss_const_best=const_loss(c2), loss_smag_021=const_loss(0.21 ** 2), best_const_coeff=c2)
+               dt=dt, steps=nsteps, fine_seconds=el, samples=n, Delta=Delta, train_steps=a.train_steps,
+               batch=a.batch, nu_floor=nu, models={})
     os.makedirs(a.out, exist_ok=True)
-    mlp.save(os.path.join(a.out, "closure.npz"))
+    for v, table in tables.items():
+        tab = table[:, :n]
+        rms = float(torch.sqrt((tab[:9].double() ** 2).mean()).item())
+        mlp0 = closure.MLPParams.random(seed=a.seed, y0=2 * math.log(0.1),
+                                        shift=np.zeros(10), scale=np.r_[np.full(9, 1.0 / rms), 1.0 / Delta])
+        P, Qv, R = (tab[10].double(), tab[11].double(), tab[12].double())
+        c2 = float((-(Qv.sum()) / P.sum()).item())
+        log(f"[train] {v}: {n} samples; grad-u rms {rms:.3e} 1/s; Delta {Delta:.4e} m; best constant exp(y) "
+            f"{c2:.4e} (C_s {math.sqrt(max(c2, 0)):.3f})")
+        tr = closure.ClosureTrainer(mlp0, max_batch=a.batch)
+        inv_norm = 1.0 / float(R.mean().item())
+        t0 = time.time()
+        hist = tr.fit(table, n, steps=a.train_steps, batch=a.batch, lr=a.lr, lr_final=a.lr / 30,
+                      inv_norm=inv_norm, seed=a.seed, log=log, log_every=max(1, a.train_steps // 10))
+        torch.cuda.synchronize()
+        t_train = time.time() - t0
+
+        def const_loss(e):
+            return float(((e * e * P + 2 * e * Qv + R).mean() / R.mean()).item())
+
+        # loss of the trained network over the whole table, in batches
+        tot = 0.0
+        for off in range(0, n, a.batch):
+            b = min(a.batch, n - off)
+            idx = torch.arange(off, off + b, device="cuda", dtype=torch.int64)
+            tot += float(tr.loss_grad(table, b, idx, inv_norm / n)[closure.NPARAM].item())
+        mlp = tr.mlp()
+        mlp.save(os.path.join(a.out, f"closure_{v}.npz"))
+        res["models"][v] = dict(grad_rms=rms, train_seconds=t_train, history=hist, loss_trained=tot,
+                                loss_const_best=const_loss(max(c2, 0.0)), loss_no_network=const_loss(0.0),
+                                loss_smag_021=const_loss(0.21 ** 2), best_const_coeff=c2)
+        log(f"[train] {v} done: loss {tot:.4e} (best constant {const_loss(max(c2, 0.0)):.4e}, floor only "
+            f"{const_loss(0.0):.4e}, C_s=0.21 {const_loss(0.21 ** 2):.4e}), {t_train:.1f} s")
     with open(os.path.join(a.out, "train.json"), "w") as f:
         json.dump(res, f, indent=1)
-    log(f"[train] done: loss {hist[-1][1]:.4e} (best constant {res['loss_const_best']:.4e}, "
-        f"C_s=0.21 {res['loss_smag_021']:.4e}), {t_train:.1f} s")
 
 
 def stage_eval(a):
     prm, fine, U0, nu = setup(a.fine, a.mach, a.re)
     _, coarse, _, _ = setup(a.coarse, a.mach, a.re)
-    mlp = closure.MLPParams.load(a.weights)
+    nets = {}
+    for path in (a.weights.split(",") if a.weights else []):
+        nets["learned_" + os.path.splitext(os.path.basename(path))[0].replace("closure_", "")] = \
+            closure.MLPParams.load(path)
+    if a.fitted_cs > 0:
+        nets["fitted_const"] = closure.MLPParams.constant(2 * math.log(a.fitted_cs))
+    nets["smag021_floor"] = closure.MLPParams.constant(2 * math.log(0.21))
     dt_f = time_step(fine, U0, a.courant, prm)
     tend = a.tend / U0
     nf = int(math.ceil(tend / dt_f))
@@ -201,11 +221,9 @@ def stage_eval(a):
     torch.cuda.empty_cache()
     out = dict(stage="eval", fine=a.fine, coarse=a.coarse, U0=U0, nu=nu, tend=a.tend, dt_fine=dt_f,
                steps_fine=nf, steps_coarse=nc, fine_seconds=el_f, points_per_element=p, models={})
-    for kind in a.models.split(","):
-        if kind == "fitted" and a.fitted_cs <= 0:
-            continue
-        net = closure.MLPParams.constant(2 * math.log(a.fitted_cs)) if kind == "fitted" else mlp
-        dgc = model(prm, coarse, kind, nu, net)
+    kinds = [k for k in a.models.split(",") if k != "nets"] + (list(nets) if "nets" in a.models.split(",") else [])
+    for kind in kinds:
+        dgc = model(prm, coarse, "net", nu, nets[kind]) if kind in nets else model(prm, coarse, kind, nu)
         Qc = Qc0.clone()
         res = {}
 
@@ -250,13 +268,14 @@ def main(argv=None):
     ap.add_argument("--phase", type=int, default=1, help="seeded phase shift of the training vortex")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--train-steps", type=int, default=20000)
-    ap.add_argument("--average", type=int, default=1, help="element-averaged P, Q, R targets")
     ap.add_argument("--fitted-cs", type=float, default=0.0,
                     help="also evaluate the constant network of this coefficient (model 'fitted')")
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--weights", default=None)
-    ap.add_argument("--models", default="learned,fitted,smag,smag64,none")
+    ap.add_argument("--models", default="nets,smag,smag64,none",
+                    help="'nets' = every network given (--weights, --fitted-cs) and Smagorinsky C_s = 0.21 "
+                         "with the molecular floor")
     ap.add_argument("--eval-times", default="4,6,8")
     ap.add_argument("--points", type=int, default=4)
     ap.add_argument("--out", required=True)

@@ -724,8 +724,8 @@ int cmdg_box_filter(const double *Qf, const int32_t *children, int64_t ncoarse, 
 }
 
 int cmdg_closure_samples(const double *moments, const double *grad_u, int64_t nelem, int32_t N, double Delta,
-                         double nu_mol, int32_t element_average, float *samples, int64_t ld, int64_t offset,
-                         void *stream) {
+                         double nu_mol, double nu_floor, int32_t element_average, float *samples, int64_t ld,
+                         int64_t offset, void *stream) {
     if ((N != 3 && N != 4) || nelem < 0 || offset < 0) return CMDG_EINVAL;
     const int Np = (N + 1) * (N + 1) * (N + 1);
     const int64_t n = nelem * Np;
@@ -733,7 +733,7 @@ int cmdg_closure_samples(const double *moments, const double *grad_u, int64_t ne
     if (n == 0) return CMDG_OK;
     if (!moments || !grad_u || !samples) return CMDG_EINVAL;
     hipLaunchKernelGGL(k_closure_samples, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, moments,
-                       grad_u, nelem, Np, Delta, nu_mol, samples, ld, offset);
+                       grad_u, nelem, Np, Delta, nu_mol, nu_floor, samples, ld, offset);
     if (hipGetLastError() != hipSuccess) return CMDG_EHIP;
     if (element_average)
         hipLaunchKernelGGL(k_element_average_pqr, dim3((unsigned)nelem), dim3(128), 0, (hipStream_t)stream, samples, ld,

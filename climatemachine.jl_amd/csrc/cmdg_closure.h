@@ -6,8 +6,10 @@
 // index 3c + d, and the filter width Delta at index 9; the network sees
 // x = (raw - shift) * scale.  The output y is the logarithm of the eddy viscosity
 // in units of Delta^2 |S| (|S| = sqrt(2 S:S)):
-//     nu_0 = exp(min(y, MLP_YMAX)) * Delta^2 * |S| + 1e-5,
-// which takes the place of nu_0 = |S| (C_s Delta)^2 + 1e-5 of Smagorinsky-Lilly
+//     nu_0 = exp(min(y, MLP_YMAX)) * Delta^2 * |S| + nu_floor,
+// nu_floor the context's kinematic viscosity (the molecular viscosity of a flow at
+// finite Reynolds number), 1e-5 when that is zero.  It takes the place of
+// nu_0 = |S| (C_s Delta)^2 + 1e-5 of Smagorinsky-Lilly
 // (src/Common/TurbulenceClosures/TurbulenceClosures.jl:478-491); the buoyancy
 // correction and the stress/diffusivity that follow are Smagorinsky-Lilly's.  A
 // network whose output is the constant 2 log C_s reproduces Smagorinsky-Lilly.
@@ -130,9 +132,9 @@ __device__ __forceinline__ void mlp_inputs(const MlpDev *__restrict__ W, const f
     x[9] = (Delta - W->shift[9]) * W->scale[9];
 }
 
-__device__ __forceinline__ float mlp_nu0(float y, float Delta, float normS) {
+__device__ __forceinline__ float mlp_nu0(float y, float Delta, float normS, float nu_floor) {
 #pragma clang fp contract(fast)
-    return __expf(fminf(y, MLP_YMAX)) * (Delta * Delta) * normS + 1e-5f;
+    return __expf(fminf(y, MLP_YMAX)) * (Delta * Delta) * normS + nu_floor;
 }
 
 // Stand-alone evaluation of the network on n raw input rows (row-major n x 10):

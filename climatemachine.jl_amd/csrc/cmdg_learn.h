@@ -142,15 +142,16 @@ k_box_filter(const double *__restrict__ Qf, const int32_t *__restrict__ children
 // velocity gradient Gu (Np, 9, E):
 //   u = (rho u)/rho,  tau_ab = (rho u_a u_b)/rho - u_a u_b   (subgrid stress),
 //   S = sym(Gu), S^d its deviator, |S| = sqrt(2 S:S), c = Delta^2 |S|,
-//   T = tau^d - 2 (nu_mol - 1e-5) S^d   (what nu_0 - 1e-5 must supply),
-//   a = 2 c S^d;  the model's deviatoric stress is -exp(y) a - 2e-5 S^d, so the
-//   squared error is exp(2y) P + 2 exp(y) Q + R with P = a:a, Q = a:T, R = T:T.
+//   T = tau^d - 2 (nu_mol - nu_floor) S^d   (what nu_0 - nu_floor must supply),
+//   a = 2 c S^d;  the model's deviatoric stress is -exp(y) a - 2 nu_floor S^d, so
+//   the squared error is exp(2y) P + 2 exp(y) Q + R with P = a:a, Q = a:T, R = T:T.
 // Row layout (SoA, leading dimension ld): rows 0-8 du_c/dx_d at 3c + d, row 9
 // Delta, rows 10-12 P, Q, R.
 constexpr int SAMPLE_W = 13;
 
 __global__ void k_closure_samples(const double *__restrict__ M, const double *__restrict__ Gu, int64_t nelem, int Np,
-                                  double Delta, double nu_mol, float *__restrict__ out, int64_t ld, int64_t offset) {
+                                  double Delta, double nu_mol, double nu_floor, float *__restrict__ out, int64_t ld,
+                                  int64_t offset) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nelem * Np) return;
     const int64_t e = t / Np;
@@ -177,7 +178,7 @@ __global__ void k_closure_samples(const double *__restrict__ M, const double *__
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b) {
             const double Sd = S[a][b] - (a == b ? trS : 0.0);
-            const double T = (tau[a][b] - (a == b ? trT : 0.0)) - 2.0 * (nu_mol - 1e-5) * Sd;
+            const double T = (tau[a][b] - (a == b ? trT : 0.0)) - 2.0 * (nu_mol - nu_floor) * Sd;
             const double av = 2.0 * c * Sd;
             P += av * av;
             Qv += av * T;

@@ -255,7 +255,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 #pragma unroll
             for (int c = 0; c < MLP_IN; ++c) x[c] = 0.0f;
         const float y = mlp_wave(mlp, x, sX);
-        gf[10] = mlp_nu0(y, (float)ge[10], strain_norm_f(gf));
+        gf[10] = mlp_nu0(y, (float)ge[10], strain_norm_f(gf), MP.visc > 0.0f ? MP.visc : 1e-5f);
     }
     if (act) {
         float *go = gfo + (int64_t)Np * NGF * e + n;

@@ -16,7 +16,8 @@
 // and Smagorinsky-Lilly closure on linear velocity fields.  Smagorinsky-Lilly
 // has no reference known-answer value; the closed-form test is its only pin.
 // The learned closure (no reference counterpart) replaces Smagorinsky-Lilly's
-// nu_0 by exp(min(y, -1)) Delta^2 |S| + 1e-5, y the output of a 10-32-32-1 SiLU network on
+// nu_0 by exp(min(y, -1)) Delta^2 |S| + nu_floor (the kinematic viscosity of the
+// descriptor, 1e-5 when zero), y the output of a 10-32-32-1 SiLU network on
 // the assembled velocity gradient; its inference here emulates the BF16 rounding
 // of the GPU (inputs, weights, first hidden layer) with exact arithmetic otherwise.
 // Pinned by: a constant-output network reproduces Smagorinsky-Lilly, and the
@@ -409,7 +410,8 @@ template <class T> void learned_viscosity(Ctx<T> &c, const T *aux) {
             const double S0 = at(3), S1 = at(4), S2 = at(5), S3 = at(6), S4 = at(7), S5 = at(8);
             const double norm2 = S0 * S0 + S3 * S3 + S5 * S5 + 2.0 * (S1 * S1 + S2 * S2 + S4 * S4);
             const double D = (double)a.Delta;
-            at(19) = (T)(std::exp(y) * (D * D) * std::sqrt(2.0 * norm2) + 1e-5);
+            const double floor = c.visc > T(0) ? (double)(float)c.visc : (double)1e-5f;
+            at(19) = (T)(std::exp(y) * (D * D) * std::sqrt(2.0 * norm2) + floor);
         }
 }
 

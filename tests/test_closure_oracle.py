@@ -113,6 +113,12 @@ def test_constant_network_is_smagorinsky():
     Delta = aux[:, cs.layout["Delta"], :]
     nu0 = normS * (0.23 * Delta) ** 2 + 1e-5
     assert np.allclose(gl[:, 19], nu0, rtol=3e-7, atol=0)
+    # with a kinematic viscosity in the descriptor the closure's floor is that viscosity
+    cv = atmos.AtmosConfig(prm, turbulence=abi.LEARNED_CLOSURE, viscosity=0.3, **kw)
+    ov = O.Oracle(cv, grid, aux)
+    ov.set_mlp(closure.MLPParams.constant(2 * math.log(0.23)))
+    ov.tendency(Q)
+    assert np.allclose(ov.gradient_flux()[:, 19], normS * (0.23 * Delta) ** 2 + np.float32(0.3), rtol=3e-7, atol=0)
     # the velocity gradient the network sees: its symmetric part is S
     G = gl[:, 10:19]
     assert np.allclose(0.5 * (G[:, 1] + G[:, 3]), S[:, 1], rtol=0, atol=1e-12 * np.abs(S).max())
