@@ -274,3 +274,38 @@ def test_courant_and_weightedsum(name):
         got, ref = dg.weightedsum(Q, s), orc.weightedsum(Q0, s)
         assert abs(got - ref) <= 2e-16 * max(abs(ref), 1e-300) + 1e-300, (s, got, ref)
     dg.close()
+
+
+def test_config1_vortex_100_steps():
+    """The reference's CPU-runnable case: isentropic vortex, 10 m periodic box of
+    8 x 8 x 2 elements, N = 4, rho = 1, u = (1, 1, 0), T = 300 K, vortex strength 5,
+    seeded +-1e-4 density noise, Rusanov, 100 fused LSRK54 steps at acoustic CFL 0.8,
+    against 100 oracle steps."""
+    import torch
+    from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
+    print("[parity] config-1 vortex, 100 steps", flush=True)
+    prm = atmos.ideal_gas_parameters()
+    grid = uniform_brick((8, 8, 2), (-5.0, -5.0, -5.0), (5.0, 5.0, 5.0), N=4)
+    cfg = atmos.AtmosConfig(prm, numerical_flux=abi.RUSANOV)
+    aux = atmos.init_aux(cfg, grid)
+    setup = atmos.IsentropicVortex(p_inf=1.0 * prm.R_d * 300.0, T_inf=300.0, translation_speed=math.sqrt(2),
+                                   vortex_speed=5.0, vortex_radius=1.0, domain_halflength=5.0)
+    Q0 = setup(prm, grid)
+    Q0[:, 0, :] *= 1 + 1e-4 * np.random.default_rng(1).uniform(-1, 1, Q0[:, 0, :].shape)
+    c = math.sqrt(prm.cp_d / prm.cv_d * prm.R_d * 300.0)
+    dt = 0.8 * grid.min_node_distance() / c
+    assert abs(dt - 4.97e-4) < 1e-6   # acoustic CFL 0.8 on the smallest LGL gap
+    orc = O.Oracle(cfg, grid, aux)
+    Qr = np.asfortranarray(Q0.copy())
+    dQr = np.zeros_like(Qr, order="F")
+    for k in range(100):
+        orc.lsrk54_step(Qr, dQr, k * dt, dt)
+    dg = DGModel(cfg, grid, aux=aux)
+    Q = to_device(Q0)
+    solve(Q, LSRK54CarpenterKennedy(dg, Q, dt=dt), numberofsteps=100, timeend=math.inf)
+    got = to_host(Q)
+    for s in range(5):
+        r = rel_l2(got[:, s] - Q0[:, s], Qr[:, s] - Q0[:, s])
+        print(f"  state {s}: rel L2 of the change {r:.3e}", flush=True)
+        assert r < 1e-10, (s, r)
+    dg.close()
