@@ -167,9 +167,8 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     double(*sJ)[NGF] = reinterpret_cast<double(*)[NGF]>(sJbuf);
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
-    int32_t nb[6];  // the element's face neighbours (uniform: scalar loads, issued first)
-#pragma unroll
-    for (int f = 0; f < 6; ++f) nb[f] = A.nbe[6 * e + f];
+    __shared__ int32_t nb[6];  // the element's face neighbours
+    if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
     if (n < Nq * Nq) sD[n] = A.D[n];
     const bool act = n < Np;
@@ -203,7 +202,6 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     }
     __syncthreads();
 
-    int fl[3], fnl[3], nf = 0;
     double gf[10];
     if (act) {
         // horizontal pass then vertical pass, transform applied to each and summed
@@ -235,11 +233,14 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
         for (int s = 0; s < NGF; ++s) gf[s] += tmp[s];
         // the lifted jumps of this node's faces, in face order: M^-1 Mf (...)
         const double MI = 1.0 / (ge[3] * A.W[n]);
-        nf = node_faces<Nq>(i, j, k, fl, fnl);
-        for (int c = 0; c < nf; ++c) {
-            const double w = MI * (ge[4 + fl[c]] * A.Wf[fnl[c]]);
 #pragma unroll
-            for (int s = 0; s < NGF; ++s) gf[s] += w * sJ[fl[c] * Nfp + fnl[c]][s];
+        for (int d = 0; d < 3; ++d) {
+            int fn;
+            const int f = node_face_dir<Nq>(i, j, k, d, fn);
+            if (f < 0) continue;
+            const double w = MI * (ge[4 + f] * A.Wf[fn]);
+#pragma unroll
+            for (int s = 0; s < NGF; ++s) gf[s] += w * sJ[f * Nfp + fn][s];
         }
         double *go = gfo + (int64_t)Np * NGF * e + n;
 #pragma unroll
@@ -249,14 +250,17 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     // written to the trace face-major
     __syncthreads();
     double(*sF2)[TR_F2][Np] = reinterpret_cast<double(*)[TR_F2][Np]>(sJbuf);
-    if (act && nf > 0) {
+    if (act && (i == 0 || i == Nq - 1 || j == 0 || j == Nq - 1 || k == 0 || k == Nq - 1)) {
         double nu[3];
         viscosity<SMAG>(P, q[0], gf, a.gPhi, ge[10], nu);
-        for (int c = 0; c < nf; ++c) {
-            double F2[5];
-            flux2_row<SMAG>(P, q, gf, nu, fl[c] >> 1, F2);
 #pragma unroll
-            for (int v = 0; v < TR_F2; ++v) sF2[fl[c] >> 1][v][n] = F2[1 + v];
+        for (int d = 0; d < 3; ++d) {
+            int fn;
+            if (node_face_dir<Nq>(i, j, k, d, fn) < 0) continue;
+            double F2[5];
+            flux2_row<SMAG>(P, q, gf, nu, d, F2);
+#pragma unroll
+            for (int v = 0; v < TR_F2; ++v) sF2[d][v][n] = F2[1 + v];
         }
     }
     __syncthreads();
@@ -306,8 +310,8 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
     for (int s = 0; s < 5; ++s) flux[s] = (Fm[s] + Fp[s]) * h;
     if (P.rusanov) {
         const double rim = 1.0 / q[0], rip = 1.0 / qp[0];
-        const double qmd = d == 0 ? q[1] : (d == 1 ? q[2] : q[3]);
-        const double qpd = d == 0 ? qp[1] : (d == 1 ? qp[2] : qp[3]);
+        const double qmd = pick3(d, q[1], q[2], q[3]);
+        const double qpd = pick3(d, qp[1], qp[2], qp[3]);
         const double wm = fabs(nd * (rim * qmd)) + sqrt(P.gammaR * tm.temp);
         const double wp = fabs(nd * (rip * qpd)) + sqrt(P.gammaR * tp.temp);
         const double mw = fmax(wm, wp);
@@ -343,9 +347,8 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     __shared__ double sFl[6 * Nfp][5];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
-    int32_t nb[6];  // the element's face neighbours (uniform: scalar loads, issued first)
-#pragma unroll
-    for (int f = 0; f < 6; ++f) nb[f] = A.nbe[6 * e + f];
+    __shared__ int32_t nb[6];  // the element's face neighbours
+    if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
     if (n < Nq * Nq) sD[n] = A.D[n];
     const bool act = n < Np;
@@ -438,10 +441,11 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
         (void)u;
         (void)pm;
         // the face fluxes of this node's faces, in face order
-        int fl[3], fnl[3];
-        const int nf = node_faces<Nq>(i, j, k, fl, fnl);
-        for (int c = 0; c < nf; ++c) {
-            const int f = fl[c], fn = fnl[c];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int fn;
+            const int f = node_face_dir<Nq>(i, j, k, d, fn);
+            if (f < 0) continue;
             const double w = (MODE == 0 ? alpha * MI : MI) * (ge[4 + f] * A.Wf[fn]);
 #pragma unroll
             for (int s = 0; s < 5; ++s) T[s] -= w * sFl[f * Nfp + fn][s];

@@ -23,8 +23,8 @@ struct MixParams {
 
 __device__ __forceinline__ void flux1_row_f(const float q[5], const float u[3], float p, float pm, int d, float F[5]) {
 #pragma clang fp contract(fast)
-    const float qd = d == 0 ? q[1] : (d == 1 ? q[2] : q[3]);
-    const float ud = d == 0 ? u[0] : (d == 1 ? u[1] : u[2]);
+    const float qd = pick3(d, q[1], q[2], q[3]);
+    const float ud = pick3(d, u[0], u[1], u[2]);
     F[0] = qd;
     F[1] = qd * u[0] + (d == 0 ? pm : 0.0f);
     F[2] = qd * u[1] + (d == 1 ? pm : 0.0f);
@@ -69,10 +69,10 @@ __device__ __forceinline__ void flux2_row_f(const MixParams &P, const float q[5]
                                             int d, float F[5]) {
 #pragma clang fp contract(fast)
     const float *S = gf + 3;
-    const float S0 = d == 0 ? S[0] : (d == 1 ? S[1] : S[2]);
-    const float S1 = d == 0 ? S[1] : (d == 1 ? S[3] : S[4]);
-    const float S2 = d == 0 ? S[2] : (d == 1 ? S[4] : S[5]);
-    const float nud = d == 0 ? nu[0] : (d == 1 ? nu[1] : nu[2]);
+    const float S0 = pick3(d, S[0], S[1], S[2]);
+    const float S1 = pick3(d, S[1], S[3], S[4]);
+    const float S2 = pick3(d, S[2], S[4], S[5]);
+    const float nud = pick3(d, nu[0], nu[1], nu[2]);
     const float m2 = -2.0f * nud;
     float t0 = m2 * S0, t1 = m2 * S1, t2 = m2 * S2;
     if (!SMAG && P.with_div) {
@@ -85,7 +85,7 @@ __device__ __forceinline__ void flux2_row_f(const MixParams &P, const float q[5]
     F[1] = t0 * q[0];
     F[2] = t1 * q[0];
     F[3] = t2 * q[0];
-    const float gd = d == 0 ? gf[0] : (d == 1 ? gf[1] : gf[2]);
+    const float gd = pick3(d, gf[0], gf[1], gf[2]);
     F[4] = t0 * q[1] + t1 * q[2] + t2 * q[3] - (nud * P.inv_Pr) * gd * q[0];
 }
 
@@ -149,9 +149,8 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     __shared__ uint16_t sX[LEARN ? (NT / 64) * MLP_LDS_PER_WAVE : 1];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
-    int32_t nb[6];  // the element's face neighbours (uniform: scalar loads, issued first)
-#pragma unroll
-    for (int f = 0; f < 6; ++f) nb[f] = A.nbe[6 * e + f];
+    __shared__ int32_t nb[6];  // the element's face neighbours
+    if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
     if (n < Nq * Nq) sD[n] = (float)A.D[n];
     const bool act = n < Np;
@@ -205,7 +204,9 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
             double qb[5];
 #pragma unroll
             for (int s = 0; s < 5; ++s) qb[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
-            qb[1 + d] = 0.0;  // the gradient wall state removes the normal momentum
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (c == d) qb[1 + c] = 0.0;  // the gradient wall state removes the normal momentum
             grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, A.str[tri<Nfp>(e, f, 0, TR_S, fn)]), Gp);
 #pragma unroll
             for (int s = 0; s < 5; ++s) jump[s] = nd * (Gp[s] - Gm[s]);
@@ -217,7 +218,6 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 
     const float xi0 = (float)ge[0], xi1 = (float)ge[1], xi2 = (float)ge[2];
     float g[3][5] = {};
-    int fl[3], fnl[3], nf = 0;
     float gf[11];
     if (act) {
 #pragma unroll
@@ -237,9 +237,11 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         }
         // lifted central jumps of this node's faces: M^-1 Mf n (G* - G-)
         const float MI = (float)(1.0 / (ge[3] * A.W[n]));
-        nf = node_faces<Nq>(i, j, k, fl, fnl);
-        for (int c = 0; c < nf; ++c) {
-            const int f = fl[c], d = f >> 1, fn = fnl[c];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int fn;
+            const int f = node_face_dir<Nq>(i, j, k, d, fn);
+            if (f < 0) continue;
             const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
 #pragma unroll
             for (int s = 0; s < 5; ++s) g[d][s] += w * sJ[f * Nfp + fn][s];
@@ -262,16 +264,19 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 #pragma unroll
         for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
         // second-order flux rows of the face nodes, staged by direction
-        if (nf > 0) {
+        if (i == 0 || i == Nq - 1 || j == 0 || j == Nq - 1 || k == 0 || k == Nq - 1) {
             float nu[3], qf[5];
 #pragma unroll
             for (int s = 0; s < 5; ++s) qf[s] = (float)q[s];
             viscosity_f<SMAG, LEARN>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
-            for (int c = 0; c < nf; ++c) {
-                float F2[5];
-                flux2_row_f<SMAG>(MP, qf, gf, nu, fl[c] >> 1, F2);
 #pragma unroll
-                for (int v = 0; v < TR_F2; ++v) sF2[fl[c] >> 1][v][n] = F2[1 + v];
+            for (int d = 0; d < 3; ++d) {
+                int fn;
+                if (node_face_dir<Nq>(i, j, k, d, fn) < 0) continue;
+                float F2[5];
+                flux2_row_f<SMAG>(MP, qf, gf, nu, d, F2);
+#pragma unroll
+                for (int v = 0; v < TR_F2; ++v) sF2[d][v][n] = F2[1 + v];
             }
         }
     }
@@ -310,7 +315,9 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     } else {
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = q[s];
-        qp[1 + d] = -q[1 + d];  // reflect the normal momentum
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (c == d) qp[1 + c] = -q[1 + c];  // reflect the normal momentum
     }
     const ThermoD tm = thermo_d(P, q, Phi), tp = thermo_d(P, qp, Php);
     float qf[5], qpf[5], uf[3], upf[3], jump[5];
@@ -332,8 +339,8 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
 #pragma unroll
     for (int s = 0; s < 5; ++s) flux[s] = (Fm[s] + Fp[s]) * h;
     if (MP.rusanov) {
-        const float umd = d == 0 ? uf[0] : (d == 1 ? uf[1] : uf[2]);
-        const float upd = d == 0 ? upf[0] : (d == 1 ? upf[1] : upf[2]);
+        const float umd = pick3(d, uf[0], uf[1], uf[2]);
+        const float upd = pick3(d, upf[0], upf[1], upf[2]);
         const float mw = 0.5f * fmaxf(fabsf(umd) + sqrtf(MP.gammaR * (float)tm.temp),
                                       fabsf(upd) + sqrtf(MP.gammaR * (float)tp.temp));
 #pragma unroll
@@ -361,9 +368,8 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     __shared__ float sFl[6 * Nfp][5];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
-    int32_t nb[6];  // the element's face neighbours (uniform: scalar loads, issued first)
-#pragma unroll
-    for (int f = 0; f < 6; ++f) nb[f] = A.nbe[6 * e + f];
+    __shared__ int32_t nb[6];  // the element's face neighbours
+    if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
     if (n < Nq * Nq) sD[n] = (float)A.D[n];
     const bool act = n < Np;
@@ -465,10 +471,11 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 #pragma unroll
             for (int d = 0; d < 3; ++d) T[1 + d] -= drho * gPhi[d];
         }
-        int fl[3], fnl[3];
-        const int nf = node_faces<Nq>(i, j, k, fl, fnl);
-        for (int c = 0; c < nf; ++c) {
-            const int f = fl[c], fn = fnl[c];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int fn;
+            const int f = node_face_dir<Nq>(i, j, k, d, fn);
+            if (f < 0) continue;
             const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
 #pragma unroll
             for (int s = 0; s < 5; ++s) T[s] -= w * sFl[f * Nfp + fn][s];

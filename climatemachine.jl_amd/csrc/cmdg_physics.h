@@ -196,6 +196,26 @@ __device__ __forceinline__ int node_faces(int i, int j, int k, int f[3], int fn[
     return c;
 }
 
+// Component d (0, 1, 2) of three values, chosen among values rather than among array
+// elements: `d == 0 ? a[0] : ...` on arrays is a choice of address, which keeps the
+// array in private (scratch) memory when d is only known at run time.
+template <typename T>
+__device__ __forceinline__ T pick3(int d, T a, T b, T c) {
+    return d == 0 ? a : (d == 1 ? b : c);
+}
+
+// The face of direction d (0, 1, 2) that node (i, j, k) lies on -- 2d or 2d + 1 -- or
+// -1, with its face-node index fn.  Looping d = 0, 1, 2 (unrolled) visits a node's
+// faces in ascending face order, the order of node_faces, with compile-time
+// indices: per-direction arrays then stay in registers instead of private memory.
+template <int Nq>
+__device__ __forceinline__ int node_face_dir(int i, int j, int k, int d, int &fn) {
+    constexpr int N = Nq - 1;
+    const int id = pick3(d, i, j, k);
+    fn = pick3(d, j + Nq * k, i + Nq * k, i + Nq * j);
+    return id == 0 ? 2 * d : (id == N ? 2 * d + 1 : -1);
+}
+
 template <int Nq> constexpr int threads_for() { return ((Nq * Nq * Nq + 63) / 64) * 64; }
 
 // Workgroups are dealt round-robin to the 8 XCDs; give each XCD one contiguous
@@ -211,8 +231,8 @@ inline int64_t xcd_grid(int64_t nelem) { return ((nelem + NXCD - 1) / NXCD) * NX
 // Row d of the first-order flux (F[d][0..4]); u = q / rho, pm = p or p - p_ref.
 __device__ __forceinline__ void flux1_row(const double q[5], const double u[3], double p, double pm, int d,
                                           double F[5]) {
-    const double qd = d == 0 ? q[1] : (d == 1 ? q[2] : q[3]);
-    const double ud = d == 0 ? u[0] : (d == 1 ? u[1] : u[2]);
+    const double qd = pick3(d, q[1], q[2], q[3]);
+    const double ud = pick3(d, u[0], u[1], u[2]);
     F[0] = qd;
     F[1] = qd * u[0] + (d == 0 ? pm : 0.0);
     F[2] = qd * u[1] + (d == 1 ? pm : 0.0);
@@ -252,10 +272,10 @@ template <bool SMAG>
 __device__ __forceinline__ void flux2_row(const KParams &P, const double q[5], const double *gf, const double nu[3],
                                           int d, double F[5]) {
     const double *S = gf + 3;
-    const double S0 = d == 0 ? S[0] : (d == 1 ? S[1] : S[2]);
-    const double S1 = d == 0 ? S[1] : (d == 1 ? S[3] : S[4]);
-    const double S2 = d == 0 ? S[2] : (d == 1 ? S[4] : S[5]);
-    const double nud = d == 0 ? nu[0] : (d == 1 ? nu[1] : nu[2]);
+    const double S0 = pick3(d, S[0], S[1], S[2]);
+    const double S1 = pick3(d, S[1], S[3], S[4]);
+    const double S2 = pick3(d, S[2], S[4], S[5]);
+    const double nud = pick3(d, nu[0], nu[1], nu[2]);
     double t0 = (-2.0 * nud) * S0, t1 = (-2.0 * nud) * S1, t2 = (-2.0 * nud) * S2;
     if (!SMAG && P.with_div) {
         const double dv = ((2.0 * nud) / 3.0) * ((S[0] + S[3]) + S[5]);
@@ -268,7 +288,7 @@ __device__ __forceinline__ void flux2_row(const KParams &P, const double q[5], c
     F[2] = t1 * q[0];
     F[3] = t2 * q[0];
     const double vf = (t0 * q[1] + t1 * q[2]) + t2 * q[3];
-    const double gd = d == 0 ? gf[0] : (d == 1 ? gf[1] : gf[2]);
+    const double gd = pick3(d, gf[0], gf[1], gf[2]);
     F[4] = vf + ((-(nud * P.inv_Pr)) * gd) * q[0];
 }
 
