@@ -18,7 +18,9 @@ synthetic initial state.  Three legs on the same initial state and steps:
 Timed region: K steps, state already resident in HBM, bracketed by device
 synchronisation.  GDOF/s per stage = DOF / (t_step / 5) / 1e9.  The roofline
 of the chosen form's dominant kernel uses per-kernel HIP-event durations recorded
-on the library's stream during a second pass of the same K steps.  The CPU
+on the library's stream during a second pass of the same K steps; its HBM
+traffic per launch comes from two rocprofv3 counter passes in child processes
+(FETCH_SIZE, WRITE_SIZE, scaled by the counters of a 1 GiB copy).  The CPU
 baseline (rank 0, N = 1) times the C++ oracle -- a restatement of the
 reference algorithm -- on a bounded sample of the plain workload.
 """
@@ -147,6 +149,72 @@ def kernel_pass(dg, Q, solver, steps):
     return {k: (ms / max(n, 1), n) for k, (ms, n) in kt.items()}
 
 
+def hbm_traffic(kernel_prefix, workload, ne, timeout_s=150):
+    """HBM bytes per launch of the kernels whose name starts with `kernel_prefix`,
+    from two rocprofv3 counter passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch)
+    over a short run of the chosen-form leg in a child process.  Each counter is
+    scaled by what it reads for a plain 1 GiB device-to-device copy run first in the
+    same child (1 GiB read and 1 GiB written).  Returns (bytes, detail) or
+    (None, reason)."""
+    import glob
+    import shutil
+    import signal
+    import sqlite3
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    cmd_tail = ["--", sys.executable, os.path.abspath(__file__), "--only", "learned", "--steps", "2",
+                "--warmup", "1", "--workload", workload, "--elements", str(ne),
+                "--copy-calibration-mib", "1024", "--traffic", "off", "--no-cpu-baseline"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    per = {}
+    with tempfile.TemporaryDirectory(prefix="cmdg_pmc_", dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = os.path.join(tmp, counter)
+            cmd = [prof, "--pmc", counter, "-d", out, "-o", counter] + cmd_tail
+            p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE,
+                                 start_new_session=True)
+            try:
+                _, err = p.communicate(timeout=timeout_s)
+            except subprocess.TimeoutExpired:
+                os.killpg(p.pid, signal.SIGKILL)
+                p.communicate()
+                return None, f"rocprofv3 --pmc {counter} did not finish in {timeout_s} s"
+            dbs = glob.glob(os.path.join(out, "**", "*.db"), recursive=True)
+            if p.returncode != 0 and not dbs:
+                tail = err.decode(errors="replace").strip().splitlines()[-1:] if err else []
+                return None, f"rocprofv3 --pmc {counter} exited {p.returncode}: {' '.join(tail)}"
+            kern, calib = [], []
+            for db in dbs:
+                con = sqlite3.connect(db)
+                rows = con.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection "
+                                   "order by dispatch_id").fetchall()
+                con.close()
+                for did, name, cname, value in rows:
+                    if cname != counter:
+                        continue
+                    bare = name.replace("void ", "").replace("cmdg::", "")
+                    if bare.startswith(kernel_prefix):
+                        kern.append(float(value))
+                    elif did <= 2:  # the calibration: a 1 GiB fill, then the 1 GiB copy
+                        calib.append(float(value))
+            if not kern or not calib or max(calib) <= 0:
+                return None, f"no {counter} records for {kernel_prefix} or for the calibration copy"
+            scale = (1 << 20) / max(calib)  # KiB counted for the 1 GiB copy -> true bytes
+            per[counter] = (sum(kern) / len(kern), scale, len(kern))
+    fetch = per["FETCH_SIZE"][0] * per["FETCH_SIZE"][1] * 1024
+    write = per["WRITE_SIZE"][0] * per["WRITE_SIZE"][1] * 1024
+    detail = {"fetch_bytes": round(fetch), "write_bytes": round(write),
+              "fetch_scale": round(per["FETCH_SIZE"][1], 4), "write_scale": round(per["WRITE_SIZE"][1], 4),
+              "dispatches": per["FETCH_SIZE"][2],
+              "method": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch), averaged over the kernel's "
+                        "dispatches in 3 steps of the chosen form, each scaled by the same counter on a 1 GiB "
+                        "device copy"}
+    return fetch + write, detail
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +226,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=("all", "fp64", "mixed", "learned"), default="all",
                     help="profile one leg alone")
+    ap.add_argument("--traffic", choices=("on", "off"), default="on",
+                    help="measure the dominant kernel's HBM bytes per launch with two rocprofv3 counter passes "
+                         "(child processes; use off when the bench itself runs under a profiler)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
@@ -251,9 +322,17 @@ def main():
         return
 
     t_avg, g_avg = kt["tendency"][0], kt["gradient"][0]
-    dom = ("k_tendency_mix (LSRK-fused, mixed)", t_avg, tend_bytes) if t_avg >= g_avg else \
-          ("k_gradient_mix (learned closure fused)", g_avg, grad_bytes)
+    dom = ("k_tendency_mix (LSRK-fused, mixed)", t_avg, tend_bytes, "k_tendency_mix") if t_avg >= g_avg else \
+          ("k_gradient_mix (learned closure fused)", g_avg, grad_bytes, "k_gradient_mix")
     achieved = dom[2] * nodes / (dom[1] * 1e-3) / 1e9
+    traffic, traffic_detail = None, "not measured (--traffic off)"
+    if args.traffic == "on":
+        print(f"[bench] HBM traffic of {dom[3]}: two rocprofv3 counter passes", file=sys.stderr, flush=True)
+        traffic, traffic_detail = hbm_traffic(dom[3], args.workload, ne)
+        if traffic is not None:
+            traffic_detail["algorithmic_bytes"] = dom[2] * nodes
+            traffic_detail["ratio_to_algorithmic"] = round(traffic / (dom[2] * nodes), 3)
+        print(f"[bench] traffic: {traffic} ({traffic_detail})", file=sys.stderr, flush=True)
     workload = (f"dry rising thermal bubble LES, {ne}^3 elements, N=4, gravity + hydrostatic reference, walls in z, "
                 "Rusanov, LSRK54CarpenterKennedy" if les else
                 f"Taylor-Green vortex, Mach 0.1, {ne}^3 elements, N=4, periodic, Rusanov, LSRK54CarpenterKennedy")
@@ -293,7 +372,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None if traffic is None else int(traffic),
+            "traffic_detail": traffic_detail,
             "algorithmic_bytes_per_node": dom[2],
             "avg_launch_us": round(dom[1] * 1e3, 2),
             "k_tendency_avg_us": round(t_avg * 1e3, 2),
