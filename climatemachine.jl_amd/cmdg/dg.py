@@ -194,7 +194,15 @@ class DGModel:
     def geometry_kind(self) -> str:
         k = C.c_int32()
         self._check(self.lib.cmdg_geometry_kind(self._ctx, C.byref(k)), "cmdg_geometry_kind")
-        return "affine" if k.value == 1 else "general"
+        return "affine" if k.value & 1 else "general"
+
+    @property
+    def static_aux_continuous(self) -> bool:
+        """True when Phi and p_ref agree bitwise across every interior face of the
+        affine brick, so the face kernels read no neighbour static trace."""
+        k = C.c_int32()
+        self._check(self.lib.cmdg_geometry_kind(self._ctx, C.byref(k)), "cmdg_geometry_kind")
+        return bool(k.value & 2)
 
     def set_kernel_timing(self, enable: bool):
         self._check(self.lib.cmdg_set_kernel_timing(self._ctx, int(bool(enable))), "cmdg_set_kernel_timing")

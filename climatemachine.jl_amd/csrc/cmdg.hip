@@ -496,6 +496,20 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
                         st[o] = P.a_Phi >= 0 ? ab[(int64_t)Np * P.a_Phi] : -0.0;
                         st[o + Nfp] = P.a_ref_p >= 0 ? ab[(int64_t)Np * P.a_ref_p] : 0.0;
                     }
+            // Phi and p_ref are functions of position in every model here; when they agree
+            // bitwise on both sides of every interior face, the face kernels take the
+            // neighbour's values from their own side and skip its static trace.
+            bool cont = true;
+            for (int64_t e = 0; e < E && cont; ++e)
+                for (int f = 0; f < 6 && cont; ++f) {
+                    const int32_t ne = nbe[6 * e + f];
+                    if (ne < 0) continue;
+                    const double *a = st.data() + (((size_t)e * 6 + f) * TR_S) * Nfp;
+                    const double *b = st.data() + (((size_t)ne * 6 + (f ^ 1)) * TR_S) * Nfp;
+                    cont = std::memcmp(a, b, sizeof(double) * TR_S * Nfp) == 0;
+                }
+            const char *tenv = std::getenv("CMDG_NEIGHBOUR_STATIC_TRACE");  // 1: always read it
+            c->AA.aux_cont = (cont && !(tenv && tenv[0] == '1')) ? 1 : 0;
             if ((rc = upload(c, &c->d_str, st.data(), st.size()))) return rc;
             for (int b = 0; b < 2; ++b)
                 if ((rc = upload<double>(c, &c->d_qtr[b], nullptr, (size_t)E * 6 * TR_Q * Nfp))) return rc;
@@ -853,7 +867,7 @@ int cmdg_weightedsum(cmdg_ctx *c, const double *Q, int32_t state, double *out) {
 
 int cmdg_geometry_kind(cmdg_ctx *c, int32_t *kind) {
     if (!c || !kind) return CMDG_EINVAL;
-    *kind = c->affine ? 1 : 0;
+    *kind = (c->affine ? 1 : 0) | (c->affine && c->AA.aux_cont ? 2 : 0);
     return CMDG_OK;
 }
 

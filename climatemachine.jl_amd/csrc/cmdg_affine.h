@@ -35,6 +35,7 @@ struct AffArrays {
     const double *Wf;     // (Nfp) w_b w_a for face node a + Nq b
     const double *D;      // (Nq, Nq)
     const double *str;    // (nelem, 6, TR_S, Nfp) static trace
+    int32_t aux_cont;     // 1: Phi and p_ref agree bitwise on both sides of every interior face
 };
 
 template <int Nfp> __device__ __forceinline__ int64_t tri(int64_t e, int f, int v, int nv, int fn) {
@@ -98,6 +99,15 @@ k_trace_aff(int64_t nelem, const double *__restrict__ Q, double *__restrict__ qt
     }
 }
 
+// A static auxiliary field (Phi, p_ref) at volume node vn of element e, or dflt when
+// the model has no such field.
+template <int Nq>
+__device__ __forceinline__ double own_aux(const KParams &P, const double *__restrict__ aux, int64_t e, int vn,
+                                          int field, double dflt) {
+    constexpr int Np = Nq * Nq * Nq;
+    return field >= 0 ? aux[(int64_t)Np * (P.naux * e + field) + vn] : dflt;
+}
+
 // ------------------------------------------------------------------ gradient
 // Lifted-jump term Phi(n G*) - Phi(n G-) of face node fn of face f of element e
 // (DGModel_kernels.jl:1471-1647, CentralNumericalFluxGradient NumericalFluxes.jl:67-123,
@@ -105,9 +115,9 @@ k_trace_aff(int64_t nelem, const double *__restrict__ Q, double *__restrict__ qt
 // element's own gradient arguments in LDS, q+ from the neighbour's state trace.
 template <int Nq, bool SMAG>
 __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
-                                                  int32_t ne, const double *__restrict__ qtr,
-                                                  const double *__restrict__ aux, const double (*sG)[Nq * Nq * Nq],
-                                                  double *dgf) {
+                                                  int32_t ne, const double *__restrict__ Q,
+                                                  const double *__restrict__ qtr, const double *__restrict__ aux,
+                                                  const double (*sG)[Nq * Nq * Nq], double *dgf) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
     const double nd = (f & 1) ? 1.0 : -1.0;
@@ -127,7 +137,7 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
         double qp[5];
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
-        const double Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+        const double Php = A.aux_cont ? own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0) : A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
         Thermo tp = thermo(P, qp, Php);
         grad_arg<SMAG>(P, qp, tp, SMAG ? theta_v(P, tp) : 0.0, Gp);
 #pragma unroll
@@ -135,8 +145,8 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
     } else {
         double q[5], qb[5];
 #pragma unroll
-        for (int s = 0; s < 5; ++s) q[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
-        const double Phi = A.str[tri<Nfp>(e, f, 0, TR_S, fn)];
+        for (int s = 0; s < 5; ++s) q[s] = Q[(int64_t)Np * (s + 5 * e) + vn];
+        const double Phi = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0);
         const double nv[3] = {d == 0 ? nd : 0.0, d == 1 ? nd : 0.0, d == 2 ? nd : 0.0};
         wall_state(q, nv, 1.0, qb);
         Thermo tb = thermo(P, qb, Phi);
@@ -196,7 +206,7 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     // lifted central jumps of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double dgf[NGF];
-        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], qtr, aux, sG, dgf);
+        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, qtr, aux, sG, dgf);
 #pragma unroll
         for (int s = 0; s < NGF; ++s) sJ[t][s] = dgf[s];
     }
@@ -277,24 +287,29 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
 // walls bc_momentum.jl:24-49 / bc_energy.jl:10-17 (zero second-order flux).
 template <int Nq, bool DIFF, bool SMAG>
 __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
-                                              int32_t ne, const double *__restrict__ qtr,
-                                              const double *__restrict__ f2tr, double flux[5]) {
-    constexpr int Nfp = Nq * Nq;
-    const int d = f >> 1;
+                                              int32_t ne, const double *__restrict__ Q, const double *__restrict__ aux,
+                                              const double *__restrict__ qtr, const double *__restrict__ f2tr,
+                                              double flux[5]) {
+    constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+    const int d = f >> 1, vn = face_node<Nq>(f, fn);
     const double nd = (f & 1) ? 1.0 : -1.0;
     const double h = nd * 0.5;
     const bool wall = ne < 0;
     const int fo = f ^ 1;
+    // the own side at the face's volume node: the block's volume phase has just read
+    // these lines, so they come from cache; the neighbour's side from its trace
     double q[5], qp[5];
 #pragma unroll
-    for (int s = 0; s < 5; ++s) q[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
-    const double Phi = A.str[tri<Nfp>(e, f, 0, TR_S, fn)], ref_p = A.str[tri<Nfp>(e, f, 1, TR_S, fn)];
+    for (int s = 0; s < 5; ++s) q[s] = Q[(int64_t)Np * (s + 5 * e) + vn];
+    const double Phi = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0), ref_p = own_aux<Nq>(P, aux, e, vn, P.a_ref_p, 0.0);
     double Php = Phi, refpp = ref_p;
     if (!wall) {
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
-        Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
-        refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
+        if (!A.aux_cont) {
+            Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+            refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
+        }
     } else {
         const double nv[3] = {d == 0 ? nd : 0.0, d == 1 ? nd : 0.0, d == 2 ? nd : 0.0};
         wall_state(q, nv, 2.0, qp);
@@ -400,7 +415,7 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double fl5[5];
-        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], qtr, f2tr, fl5);
+        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }

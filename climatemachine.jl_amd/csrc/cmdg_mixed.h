@@ -196,18 +196,19 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
             double qp[5];
 #pragma unroll
             for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
-            const double Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+            const double Php = A.aux_cont ? own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0)
+                                          : A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
             grad_arg_f<SMAG>(P, MP, qp, thermo_d(P, qp, Php), Gp);
 #pragma unroll
             for (int s = 0; s < 5; ++s) jump[s] = nd * (0.5f * (Gp[s] - Gm[s]));
         } else {
             double qb[5];
 #pragma unroll
-            for (int s = 0; s < 5; ++s) qb[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
+            for (int s = 0; s < 5; ++s) qb[s] = Q[(int64_t)Np * (s + 5 * e) + vn];
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 if (c == d) qb[1 + c] = 0.0;  // the gradient wall state removes the normal momentum
-            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, A.str[tri<Nfp>(e, f, 0, TR_S, fn)]), Gp);
+            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0)), Gp);
 #pragma unroll
             for (int s = 0; s < 5; ++s) jump[s] = nd * (Gp[s] - Gm[s]);
         }
@@ -294,24 +295,29 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 // face traces; the jump q- - q+ and the pressure perturbation in FP64).
 template <int Nq, bool DIFF>
 __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
-                                              int f, int fn, int32_t ne, const double *__restrict__ qtr,
+                                              int f, int fn, int32_t ne, const double *__restrict__ Q,
+                                              const double *__restrict__ aux, const double *__restrict__ qtr,
                                               const float *__restrict__ f2tr, double p_e, float flux[5]) {
 #pragma clang fp contract(fast)
-    constexpr int Nfp = Nq * Nq;
-    const int d = f >> 1;
+    constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+    const int d = f >> 1, vn = face_node<Nq>(f, fn);
     const float h = (f & 1) ? 0.5f : -0.5f;
     const bool wall = ne < 0;
     const int fo = f ^ 1;
+    // the own side at the face's volume node (in cache from the volume phase), the
+    // neighbour's side from its trace
     double q[5], qp[5];
 #pragma unroll
-    for (int s = 0; s < 5; ++s) q[s] = qtr[tri<Nfp>(e, f, s, TR_Q, fn)];
-    const double Phi = A.str[tri<Nfp>(e, f, 0, TR_S, fn)], ref_p = A.str[tri<Nfp>(e, f, 1, TR_S, fn)];
+    for (int s = 0; s < 5; ++s) q[s] = Q[(int64_t)Np * (s + 5 * e) + vn];
+    const double Phi = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0), ref_p = own_aux<Nq>(P, aux, e, vn, P.a_ref_p, 0.0);
     double Php = Phi, refpp = ref_p;
     if (!wall) {
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
-        Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
-        refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
+        if (!A.aux_cont) {
+            Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+            refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
+        }
     } else {
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = q[s];
@@ -447,7 +453,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         float fl5[5];
-        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], qtr, f2tr, p_e, fl5);
+        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr, p_e, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }

@@ -230,10 +230,13 @@ int cmdg_courant(cmdg_ctx *ctx, const double *Q, double dt, int32_t kind, int32_
  * src/Arrays/MPIStateArrays.jl:655-674.  Synchronises; *sum is a host double. */
 int cmdg_weightedsum(cmdg_ctx *ctx, const double *Q, int32_t state, double *sum);
 
-/* Geometry representation the context chose (diagnostic; no reference counterpart):
- * 1 = axis-aligned affine bricks (per-element metrics, detected bitwise at create),
- * 0 = general per-node metrics.  Setting CMDG_GENERAL_GEOMETRY=1 in the environment
- * before cmdg_create forces 0. */
+/* Geometry representation the context chose (diagnostic; no reference counterpart),
+ * a bit set: bit 0 = axis-aligned affine bricks (per-element metrics, detected bitwise
+ * at create; clear = general per-node metrics); bit 1 = on those bricks, Phi and p_ref
+ * agree bitwise on both sides of every interior face, so the face kernels take the
+ * neighbour's values from their own side instead of its static face trace.  Setting
+ * CMDG_GENERAL_GEOMETRY=1 in the environment before cmdg_create clears bit 0;
+ * CMDG_NEIGHBOUR_STATIC_TRACE=1 clears bit 1. */
 int cmdg_geometry_kind(cmdg_ctx *ctx, int32_t *kind);
 
 /* Per-kernel device timing (diagnostic; no reference counterpart): when enabled,

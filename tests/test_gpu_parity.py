@@ -310,3 +310,33 @@ def test_config1_vortex_100_steps():
         print(f"  state {s}: rel L2 of the change {r:.3e}", flush=True)
         assert r < 1e-10, (s, r)
     dg.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_neighbour_static_trace_paths_agree(precision):
+    """Phi and p_ref of the LES brick agree bitwise across faces, so the face kernels
+    take the neighbour's values from their own side; reading the neighbour's static
+    face trace instead (CMDG_NEIGHBOUR_STATIC_TRACE=1) must give the same state bit
+    for bit after three fused LSRK54 steps."""
+    import os
+    from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
+    print(f"[parity] neighbour static trace {precision}", flush=True)
+    cfg, grid, aux, Q0 = _case("les_walls")
+    prec = abi.FP64 if precision == "fp64" else abi.MIXED
+    out = {}
+    for forced in (False, True):
+        if forced:
+            os.environ["CMDG_NEIGHBOUR_STATIC_TRACE"] = "1"
+        try:
+            dg = DGModel(cfg, grid, aux=aux, precision=prec)
+        finally:
+            os.environ.pop("CMDG_NEIGHBOUR_STATIC_TRACE", None)
+        assert dg.geometry_kind == "affine"
+        assert dg.static_aux_continuous == (not forced)
+        Q = to_device(Q0)
+        solver = LSRK54CarpenterKennedy(dg, Q, dt=5e-3, t0=0.0)
+        solve(Q, solver, numberofsteps=3, timeend=math.inf)
+        out[forced] = to_host(Q)
+        dg.close()
+    assert np.isfinite(out[False]).all()
+    assert np.array_equal(out[False], out[True])
