@@ -15,8 +15,8 @@ synthetic initial state.  Three legs on the same initial state and steps:
 --workload tgv96 runs the decaying Taylor-Green vortex on 96^3 elements
 (110.6 M nodes) instead: FP64 Smagorinsky-Lilly against mixed + learned.
 
-Timed region: K steps, state already resident in HBM, bracketed by device
-synchronisation.  GDOF/s per stage = DOF / (t_step / 5) / 1e9.  The roofline
+Timed region: K steps as one solve!(Q, solver; numberofsteps = K) without
+callbacks, state already resident in HBM, bracketed by device synchronisation.  GDOF/s per stage = DOF / (t_step / 5) / 1e9.  The roofline
 of the chosen form's dominant kernel uses per-kernel HIP-event durations recorded
 on the library's stream during a second pass of the same K steps; its HBM
 traffic per launch comes from two rocprofv3 counter passes in child processes
@@ -120,17 +120,16 @@ def cpu_baseline(grid, cfg, aux, Q0, dt, steps, label):
 
 
 def timed(dg, Q, dt, steps, warmup):
+    """warmup, then `steps` timed steps, each run as solve!(Q, solver; numberofsteps)
+    without callbacks."""
     import torch
-    from cmdg.dg import LSRK54CarpenterKennedy
+    from cmdg.dg import LSRK54CarpenterKennedy, solve
     solver = LSRK54CarpenterKennedy(dg, Q, dt=dt, t0=0.0)
-    for _ in range(warmup):
-        solver.dostep(Q, None, solver.t)
-        solver.t += dt
+    if warmup > 0:
+        solve(Q, solver, numberofsteps=warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        solver.dostep(Q, None, solver.t)
-        solver.t += dt
+    solve(Q, solver, numberofsteps=steps)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     if not torch.isfinite(Q).all():
@@ -140,10 +139,9 @@ def timed(dg, Q, dt, steps, warmup):
 
 
 def kernel_pass(dg, Q, solver, steps):
+    from cmdg.dg import solve
     dg.set_kernel_timing(True)
-    for _ in range(steps):
-        solver.dostep(Q, None, solver.t)
-        solver.t += solver.dt
+    solve(Q, solver, numberofsteps=steps)
     kt = dg.kernel_timing()
     dg.set_kernel_timing(False)
     return {k: (ms / max(n, 1), n) for k, (ms, n) in kt.items()}

@@ -66,7 +66,7 @@ class CmdgDesc(C.Structure):
 
 
 EXPORTED = (
-    "cmdg_create", "cmdg_destroy", "cmdg_rhs", "cmdg_lsrk54_step",
+    "cmdg_create", "cmdg_destroy", "cmdg_rhs", "cmdg_lsrk54_step", "cmdg_lsrk54_steps",
     "cmdg_lsrk_stage", "cmdg_lsrk_update", "cmdg_aux", "cmdg_gradient_flux",
     "cmdg_gradient_flux_f32", "cmdg_mlp_set_weights", "cmdg_mlp_forward", "cmdg_velocity_gradient",
     "cmdg_box_filter", "cmdg_closure_samples", "cmdg_mlp_loss_grad", "cmdg_mlp_adam", "cmdg_interpolate_equispaced",
@@ -96,6 +96,7 @@ def load_library(path: str = None):
     lib.cmdg_destroy.argtypes = [vp]
     lib.cmdg_rhs.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_double]
     lib.cmdg_lsrk54_step.argtypes = [vp, vp, vp, C.c_double, C.c_double]
+    lib.cmdg_lsrk54_steps.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_int64]
     lib.cmdg_lsrk_stage.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double]
     lib.cmdg_lsrk_update.argtypes = [vp, vp, C.c_int64, C.c_double, C.c_double, C.c_double, vp]
     lib.cmdg_aux.argtypes = [vp, C.POINTER(vp)]

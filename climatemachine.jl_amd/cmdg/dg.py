@@ -168,6 +168,10 @@ class DGModel:
         self._check(self.lib.cmdg_lsrk54_step(self._ctx, C.c_void_p(_ptr(Q)), C.c_void_p(_ptr(dQ)), float(t),
                                               float(dt)), "cmdg_lsrk54_step")
 
+    def lsrk54_steps(self, Q, dQ, t, dt, nsteps: int):
+        self._check(self.lib.cmdg_lsrk54_steps(self._ctx, C.c_void_p(_ptr(Q)), C.c_void_p(_ptr(dQ)), float(t),
+                                               float(dt), int(nsteps)), "cmdg_lsrk54_steps")
+
     def lsrk_stage(self, Q, dQ, t_stage, dt, rka_next, rkb):
         self._check(self.lib.cmdg_lsrk_stage(self._ctx, C.c_void_p(_ptr(Q)), C.c_void_p(_ptr(dQ)),
                                              float(t_stage), float(dt), float(rka_next), float(rkb)),
@@ -286,6 +290,15 @@ class LSRK54CarpenterKennedy:
             self.rhs(self.dQ, Q, p, time + RKC[s] * self.dt, increment=True)
             lsrk_update(self.dQ, Q, RKA[(s + 1) % 5], RKB[s], self.dt)
 
+    def dosteps(self, Q: torch.Tensor, p, time: float, nsteps: int):
+        """nsteps calls of dostep! with the same dt and nothing in between (the
+        solve! loop without callbacks) in one library call."""
+        if isinstance(self.rhs, DGModel):
+            self.rhs.lsrk54_steps(Q, self.dQ, time, self.dt, nsteps)
+            return
+        for k in range(nsteps):
+            self.dostep(Q, p, time + k * self.dt)
+
 
 def solve(Q: torch.Tensor, solver, param=None, *, timeend: float = math.inf, adjustfinalstep: bool = True,
           numberofsteps: int = 0, callbacks: Sequence[Callable] = ()) -> float:
@@ -298,6 +311,23 @@ def solve(Q: torch.Tensor, solver, param=None, *, timeend: float = math.inf, adj
     step = 0
     time = solver.gettime()
     while time < timeend:
+        if not callbacks and hasattr(solver, "dosteps"):
+            # the whole steps ahead that no callback and no final-step adjustment
+            # separate run as one batch; time advances by the same additions
+            t, dt = solver.gettime(), solver.getdt()
+            tt, n = t, 0
+            while tt < timeend and (numberofsteps <= 0 or step + n < numberofsteps) \
+                    and not (adjustfinalstep and tt + dt > timeend):
+                tt += dt
+                n += 1
+            if n > 1:
+                solver.dosteps(Q, param, t, n)
+                step += n
+                solver.steps = step
+                time = solver.updatetime(tt)
+                if step == numberofsteps:
+                    break
+                continue
         step += 1
         solver.steps = step
         t, dt = solver.gettime(), solver.getdt()

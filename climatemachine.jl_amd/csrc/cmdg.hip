@@ -607,43 +607,63 @@ int cmdg_lsrk_stage(cmdg_ctx *c, double *Q, double *dQ, double t_stage, double d
     return rc ? fail(c, rc, "update launch") : CMDG_OK;
 }
 
-int cmdg_lsrk54_step(cmdg_ctx *c, double *Q, double *dQ, double t, double dt) {
+int cmdg_lsrk54_steps(cmdg_ctx *c, double *Q, double *dQ, double t, double dt, int64_t nsteps) {
     (void)t;
     if (!c || !Q || !dQ) return c ? fail(c, CMDG_EINVAL, "null pointer") : CMDG_EINVAL;
-    // Ping-pong between the caller's Q (C) and the context buffer (S) so that the
-    // fused stage update never overwrites a state that neighbouring elements still
-    // read: S <- C first, then the stages run S->C, C->S, S->C, C->S, S->C and the
-    // result lands in the caller's Q.  On affine bricks the state's face trace is
-    // ping-ponged alongside: written from Q by the trace kernel, then by each
-    // stage's epilogue for the next stage.
+    if (nsteps < 0) return fail(c, CMDG_EINVAL, "negative step count");
     if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
+    if (nsteps == 0) return CMDG_OK;
+    int rc;
+    if (c->affine) {
+        // Affine bricks: a stage reads its neighbours only through the state's face
+        // trace, never their volume state, so every stage updates Q in place.  The
+        // trace is ping-ponged: written from Q by the trace kernel once, then by each
+        // stage's epilogue for the next stage -- across step boundaries too, since no
+        // caller code runs between the steps of one call.
+        if ((rc = launch_trace(c, Q, c->d_qtr[0], nullptr))) return rc;
+        int cur = 0;
+        for (int64_t step = 0; step < nsteps; ++step)
+            for (int s = 0; s < 5; ++s) {
+                const double a = RKA[(s + 1) % 5], bdt = RKB[s] * dt;
+                const int wa = (s == 4);  // aux keeps the state of the last stage, as after dostep!
+                const bool last = (step == nsteps - 1) && s == 4;
+                double *qtr = c->d_qtr[cur], *qtr_out = last ? nullptr : c->d_qtr[cur ^ 1];
+                if (c->diffusive && (rc = launch_gradient(c, Q, qtr, nullptr, wa))) return rc;
+                if ((rc = launch_tendency(c, Q, qtr, dQ, 1.0, 1.0, 1, Q, qtr_out, a, bdt, c->diffusive ? 0 : wa)))
+                    return rc;
+                cur ^= 1;
+            }
+        return CMDG_OK;
+    }
+    // General geometry reads the neighbours' volume state: ping-pong between the
+    // caller's Q (C) and the context buffer (S) so that the fused stage update never
+    // overwrites a state that neighbouring elements still read: S <- C first, then
+    // the stages run S->C, C->S, S->C, C->S, S->C and the result lands in C.
     double *C = Q, *S = c->d_Qs;
     const int64_t n = (int64_t)c->Np * 5 * c->nelem;
-    int rc;
-    int cur = 0;
-    if (c->affine) {
-        if ((rc = launch_trace(c, C, c->d_qtr[0], S))) return rc;
-    } else if (!c->diffusive) {
-        if ((rc = launch_copy(c, S, C, n))) return rc;
-    }
-    double *src = S, *dst = C;
-    for (int s = 0; s < 5; ++s) {
-        const double a = RKA[(s + 1) % 5], bdt = RKB[s] * dt;
-        const int wa = (s == 4);  // aux keeps the state of the last stage, as after dostep!
-        const double *qtr = c->affine ? c->d_qtr[cur] : nullptr;
-        double *qtr_out = (c->affine && s < 4) ? c->d_qtr[cur ^ 1] : nullptr;
-        if (c->diffusive) {
-            // general geometry: the first gradient pass also leaves S <- C
-            double *Qcopy = (!c->affine && s == 0) ? S : nullptr;
-            if ((rc = launch_gradient(c, (!c->affine && s == 0) ? C : src, qtr, Qcopy, wa))) return rc;
+    for (int64_t step = 0; step < nsteps; ++step) {
+        if (!c->diffusive && (rc = launch_copy(c, S, C, n))) return rc;
+        double *src = S, *dst = C;
+        for (int s = 0; s < 5; ++s) {
+            const double a = RKA[(s + 1) % 5], bdt = RKB[s] * dt;
+            const int wa = (s == 4);
+            if (c->diffusive) {
+                // the first gradient pass also leaves S <- C
+                double *Qcopy = (s == 0) ? S : nullptr;
+                if ((rc = launch_gradient(c, (s == 0) ? C : src, nullptr, Qcopy, wa))) return rc;
+            }
+            if ((rc = launch_tendency(c, src, nullptr, dQ, 1.0, 1.0, 1, dst, nullptr, a, bdt, c->diffusive ? 0 : wa)))
+                return rc;
+            double *tmp = src;
+            src = dst;
+            dst = tmp;
         }
-        if ((rc = launch_tendency(c, src, qtr, dQ, 1.0, 1.0, 1, dst, qtr_out, a, bdt, c->diffusive ? 0 : wa))) return rc;
-        double *tmp = src;
-        src = dst;
-        dst = tmp;
-        cur ^= 1;
     }
     return CMDG_OK;
+}
+
+int cmdg_lsrk54_step(cmdg_ctx *c, double *Q, double *dQ, double t, double dt) {
+    return cmdg_lsrk54_steps(c, Q, dQ, t, dt, 1);
 }
 
 int cmdg_aux(cmdg_ctx *c, double **aux) {

@@ -348,12 +348,14 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
 
 // MODE 0: tend = alpha*RHS + beta*tend.   MODE 1: LSRK stage (tend is dQ):
 //   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka; the state trace of Qout is
-//   written to qtr_out when given.
+//   written to qtr_out when given.  Qout may be Q: a block reads only its own
+//   element's volume state (neighbours come from qtr), and all of those reads
+//   precede the workgroup barrier that comes before its writes.
 template <int Nq, bool DIFF, bool SMAG, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__ Q, const double *__restrict__ qtr,
+k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const double *__restrict__ qtr,
                const double *__restrict__ gfi, const double *__restrict__ f2tr, double *__restrict__ aux,
-               double *__restrict__ tend, double alpha, double beta, double *__restrict__ Qout,
+               double *__restrict__ tend, double alpha, double beta, double *Qout,
                double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     constexpr int NGF = SMAG ? 10 : 9;

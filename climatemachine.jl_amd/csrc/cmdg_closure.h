@@ -48,9 +48,18 @@ __host__ __device__ inline uint16_t f32_to_bf16(float f) {
     return (uint16_t)(u >> 16);
 }
 
+// BF16 rounding (nearest even) by the hardware conversion; the same value as
+// f32_to_bf16 for every finite input.
+__device__ __forceinline__ uint16_t f32_to_bf16_hw(float f) {
+    const __bf16 b = (__bf16)f;
+    return __builtin_bit_cast(uint16_t, b);
+}
+
+// SiLU v sigmoid(v) with the hardware reciprocal (1 ulp): its outputs are rounded to
+// BF16 (layer 1) or weighted by BF16 weights in an FP32 sum (layer 2).
 __device__ __forceinline__ float mlp_silu(float v) {
 #pragma clang fp contract(fast)
-    return v / (1.0f + __expf(-v));
+    return v * __builtin_amdgcn_rcpf(1.0f + __expf(-v));
 }
 
 // LDS needed per 64-lane wave: 64 rows of 16 BF16 inputs.
@@ -73,7 +82,7 @@ __device__ __forceinline__ float mlp_wave(const MlpDev *__restrict__ W, const fl
     uint32_t *row = reinterpret_cast<uint32_t *>(slab + lane * MLP_K1);
 #pragma unroll
     for (int k = 0; k < MLP_IN / 2; ++k)
-        row[k] = (uint32_t)f32_to_bf16(x[2 * k]) | ((uint32_t)f32_to_bf16(x[2 * k + 1]) << 16);
+        row[k] = (uint32_t)f32_to_bf16_hw(x[2 * k]) | ((uint32_t)f32_to_bf16_hw(x[2 * k + 1]) << 16);
 #pragma unroll
     for (int k = MLP_IN / 2; k < MLP_K1 / 2; ++k) row[k] = 0u;
     __syncthreads();
@@ -104,7 +113,7 @@ __device__ __forceinline__ float mlp_wave(const MlpDev *__restrict__ W, const fl
         for (int m = 0; m < 2; ++m) {
             const mlp_f32x4 h = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1[m], bx, B1[m], 0, 0, 0);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) hb[m][i] = (short)f32_to_bf16(mlp_silu(h[i]));
+            for (int i = 0; i < 4; ++i) hb[m][i] = (short)f32_to_bf16_hw(mlp_silu(h[i]));
         }
         float part = 0.0f;
 #pragma unroll

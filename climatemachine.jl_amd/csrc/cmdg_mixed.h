@@ -361,10 +361,10 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
 
 template <int Nq, bool DIFF, bool SMAG, bool LEARN, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
+k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *Q,
                const double *__restrict__ qtr, const float *__restrict__ gfi, const float *__restrict__ f2tr,
                double *__restrict__ aux, double *__restrict__ tend, double alpha, double beta,
-               double *__restrict__ Qout, double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
+               double *Qout, double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
 
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     constexpr int NGF = LEARN ? 11 : (SMAG ? 10 : 9), NT = threads_for<Nq>();

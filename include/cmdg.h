@@ -119,6 +119,13 @@ int cmdg_rhs(cmdg_ctx *ctx, double *tendency, const double *Q, double t,
  * coefficients of :293-327.  dQ is the solver's low-storage register. */
 int cmdg_lsrk54_step(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt);
 
+/* nsteps consecutive LSRK54CarpenterKennedy steps from t with step dt: the loop of
+ * solve!(Q, solver; numberofsteps) with no callbacks, src/Numerics/ODESolvers/
+ * ODESolvers.jl:110-158 over dostep! (LowStorageRungeKuttaMethod.jl:102-144).
+ * Identical to nsteps calls of cmdg_lsrk54_step; on affine bricks the state's face
+ * trace passes from each step's last stage to the next step's first. */
+int cmdg_lsrk54_steps(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt, int64_t nsteps);
+
 /* One low-storage stage: dQ <- RHS(Q, t_stage) + dQ; Q += rkb*dt*dQ;
  * dQ *= rka_next.  (rhs! call + update! kernel,
  * LowStorageRungeKuttaMethod.jl:122-142, 146-158) */

@@ -59,3 +59,44 @@ def test_create_rejects_bad_descriptors_without_gpu_work():
     assert b"compiled" in lib.cmdg_last_error(ctx)
     lib.cmdg_destroy(ctx)
     assert lib.cmdg_create(None, None, C.byref(ctx)) == abi.CMDG_EINVAL
+
+
+class _RecordingSolver:
+    """A solver with the ODESolvers accessors that records what solve asks of it."""
+
+    def __init__(self, dt, batches):
+        self.t, self.dt, self.steps, self.calls = 0.0, dt, 0, []
+        if batches:
+            self.dosteps = lambda Q, p, t, n: self.calls.extend(("step", t + k * self.dt, self.dt) for k in range(n))
+
+    def gettime(self):
+        return self.t
+
+    def getdt(self):
+        return self.dt
+
+    def updatedt(self, dt):
+        self.dt = dt
+
+    def updatetime(self, t):
+        self.t = t
+        return t
+
+    def dostep(self, Q, p, t):
+        self.calls.append(("step", t, self.dt))
+
+
+@pytest.mark.parametrize("kw", [dict(numberofsteps=7), dict(timeend=0.95), dict(timeend=0.95, adjustfinalstep=False),
+                                dict(timeend=0.95, numberofsteps=4), dict(numberofsteps=1)])
+def test_solve_batches_steps_like_the_step_loop(kw):
+    """solve runs the steps between callbacks and the adjusted final step as one
+    batch; the batch visits the same start times, step sizes and final time as the
+    one-step-at-a-time loop of solve! (ODESolvers.jl:110-158)."""
+    from cmdg.dg import solve
+    ref, got = _RecordingSolver(0.1, False), _RecordingSolver(0.1, True)
+    t_ref = solve(None, ref, **kw)
+    t_got = solve(None, got, **kw)
+    assert t_ref == t_got and ref.steps == got.steps and ref.dt == got.dt
+    assert [c[0] for c in got.calls] == [c[0] for c in ref.calls]
+    for a, b in zip(got.calls, ref.calls):
+        assert a[1] == pytest.approx(b[1], abs=1e-15) and a[2] == b[2]
