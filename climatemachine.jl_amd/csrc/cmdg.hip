@@ -530,7 +530,7 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
         M.R_d = (float)d->R_d;
         M.gammaR = (float)P.gammaR;
         M.kappa = (float)P.kappa;
-        M.MSLP = (float)d->MSLP;
+        M.inv_MSLP = (float)(1.0 / d->MSLP);
         M.grav = (float)d->grav;
         M.C_smag = (float)d->C_smag;
         M.inv_Pr = (float)d->inv_Pr_turb;

@@ -17,7 +17,7 @@
 namespace cmdg {
 
 struct MixParams {
-    float R_d, gammaR, kappa, MSLP, grav, C_smag, inv_Pr, visc;
+    float R_d, gammaR, kappa, inv_MSLP, grav, C_smag, inv_Pr, visc;
     int32_t dyn_visc, with_div, ref_sub, ref_src, gravity, rusanov;
 };
 
@@ -118,7 +118,9 @@ __device__ __forceinline__ ThermoD thermo_d(const KParams &P, const double q[5],
     return t;
 }
 
-// gradient argument (u, h_tot, theta_v) in FP32 from the FP64 state and thermodynamics
+// gradient argument (u, h_tot, theta_v) in FP32 from the FP64 state and thermodynamics;
+// theta_v = T (p / MSLP)^-kappa as T 2^(-kappa log2(p / MSLP)) on the hardware log2 and
+// exp2 (a few FP32 ulp: below the FP32 rounding of theta_v that its gradient sees)
 template <bool SMAG>
 __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M, const double q[5], const ThermoD &t,
                                            float G[5]) {
@@ -128,7 +130,9 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
     G[1] = (float)(rinv * q[2]);
     G[2] = (float)(rinv * q[3]);
     G[3] = (float)(q[4] * rinv + P.R_d * t.temp);
-    G[4] = SMAG ? (float)t.temp / powf((float)(t.p / P.MSLP), M.kappa) : 0.0f;
+    G[4] = SMAG ? (float)t.temp *
+                      __builtin_amdgcn_exp2f(-M.kappa * __builtin_amdgcn_logf((float)t.p * M.inv_MSLP))
+                : 0.0f;
 }
 
 // ------------------------------------------------------------------ gradient (mixed)
