@@ -89,7 +89,7 @@ struct cmdg_ctx {
     // mixed precision (FP32 fluxes and gradient flux)
     bool mixed = false;
     MixParams MP{};
-    float *d_gf32 = nullptr, *d_f2tr32 = nullptr;
+    float *d_gf32 = nullptr, *d_f2tr32 = nullptr, *d_gphi32 = nullptr;
     // learned closure (mixed precision only)
     bool learned = false, mlp_ready = false;
     MlpDev *d_mlp = nullptr;
@@ -542,6 +542,17 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
         M.gravity = P.gravity;
         M.rusanov = P.rusanov;
         if ((rc = upload<float>(c, &c->d_gf32, nullptr, (size_t)Np * c->ngf * E))) return rc;
+        if (P.a_gPhi >= 0) {
+            // grad Phi enters the mixed kernels in FP32 only: keep it rounded, 12 B a node
+            std::vector<float> g((size_t)Np * 3 * E);
+            for (int64_t e = 0; e < E; ++e)
+                for (int a = 0; a < 3; ++a)
+                    for (int n = 0; n < Np; ++n)
+                        g[((size_t)e * 3 + a) * Np + n] =
+                            (float)d->aux_init[(int64_t)Np * ((int64_t)d->naux * e + P.a_gPhi + a) + n];
+            if ((rc = upload(c, &c->d_gphi32, g.data(), g.size()))) return rc;
+            c->AA.gphi32 = c->d_gphi32;
+        }
         if ((rc = upload<float>(c, &c->d_f2tr32, nullptr, (size_t)E * 6 * TR_F2 * Nfp))) return rc;
         if (c->learned && (rc = upload<MlpDev>(c, &c->d_mlp, nullptr, 1))) return rc;
     }
@@ -571,6 +582,7 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_f2tr);
     (void)hipFree(c->d_str);
     (void)hipFree(c->d_gf32);
+    (void)hipFree(c->d_gphi32);
     (void)hipFree(c->d_f2tr32);
     (void)hipFree(c->d_mlp);
     (void)hipFree(c->d_part);

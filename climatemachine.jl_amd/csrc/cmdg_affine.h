@@ -36,6 +36,7 @@ struct AffArrays {
     const double *D;      // (Nq, Nq)
     const double *str;    // (nelem, 6, TR_S, Nfp) static trace
     int32_t aux_cont;     // 1: Phi and p_ref agree bitwise on both sides of every interior face
+    const float *gphi32;  // (Np, 3, nelem) grad Phi rounded to FP32 (mixed path; null without it)
 };
 
 template <int Nfp> __device__ __forceinline__ int64_t tri(int64_t e, int f, int v, int nv, int fn) {

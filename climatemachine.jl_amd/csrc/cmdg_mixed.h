@@ -169,10 +169,11 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
         const double *ab = aux + (int64_t)Np * P.naux * e + n;
         if (P.a_Phi >= 0) Phi = ab[(int64_t)Np * P.a_Phi];
-        if (P.a_gPhi >= 0) {
-            gPhi[0] = (float)ab[(int64_t)Np * P.a_gPhi];
-            gPhi[1] = (float)ab[(int64_t)Np * (P.a_gPhi + 1)];
-            gPhi[2] = (float)ab[(int64_t)Np * (P.a_gPhi + 2)];
+        if (A.gphi32) {
+            const float *gp = A.gphi32 + (int64_t)Np * 3 * e + n;
+            gPhi[0] = gp[0];
+            gPhi[1] = gp[Np];
+            gPhi[2] = gp[2 * Np];
         }
         const ThermoD t = thermo_d(P, q, Phi);
         grad_arg_f<SMAG>(P, MP, q, t, G);
@@ -409,10 +410,11 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         if (P.a_Phi >= 0) Phi = ab[(int64_t)Np * P.a_Phi];
         if (P.a_ref_p >= 0) ref_p = ab[(int64_t)Np * P.a_ref_p];
         double ref_rho = P.a_ref_rho >= 0 ? ab[(int64_t)Np * P.a_ref_rho] : 0.0;
-        if (P.a_gPhi >= 0) {
-            gPhi[0] = (float)ab[(int64_t)Np * P.a_gPhi];
-            gPhi[1] = (float)ab[(int64_t)Np * (P.a_gPhi + 1)];
-            gPhi[2] = (float)ab[(int64_t)Np * (P.a_gPhi + 2)];
+        if (A.gphi32) {
+            const float *gp = A.gphi32 + (int64_t)Np * 3 * e + n;
+            gPhi[0] = gp[0];
+            gPhi[1] = gp[Np];
+            gPhi[2] = gp[2 * Np];
         }
         const ThermoD t = thermo_d(P, q, Phi);
         if (write_aux) {
