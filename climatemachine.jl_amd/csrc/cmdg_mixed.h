@@ -136,8 +136,10 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 }
 
 // ------------------------------------------------------------------ gradient (mixed)
+// Five waves per SIMD: the kernel waits on memory more than it computes, and the
+// register cap this asks for (96) spills nothing at Nq = 5.
 template <int Nq, bool SMAG, bool LEARN>
-__global__ void __launch_bounds__(threads_for<Nq>())
+__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(5)))
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
                const double *__restrict__ qtr, double *__restrict__ aux, float *__restrict__ gfo,
                float *__restrict__ f2tr, int write_aux, const MlpDev *__restrict__ mlp) {
