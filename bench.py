@@ -50,6 +50,10 @@ GRADIENT_BYTES_MIXED = 40 + 24 + 44
 # the Taylor-Green workload has no Phi / reference state in aux
 TENDENCY_BYTES_MIXED_TGV = 160 + 44 + 40
 GRADIENT_BYTES_MIXED_TGV = 40 + 44
+# Per node per RK stage, both kernels (state 160, geometry 40 in FP32, aux 24, the
+# FP32 gradient flux of 11 words written and read 88); the TGV has no aux fields.
+STAGE_BYTES_MIXED = 160 + 40 + 24 + 88
+STAGE_BYTES_MIXED_TGV = 160 + 40 + 88
 WEIGHTS = os.path.join(ROOT, "climatemachine.jl_amd", "cmdg", "data", "tgv_closure.npz")
 
 
@@ -145,6 +149,33 @@ def kernel_pass(dg, Q, solver, steps):
     kt = dg.kernel_timing()
     dg.set_kernel_timing(False)
     return {k: (ms / max(n, 1), n) for k, (ms, n) in kt.items()}
+
+
+def hbm_ceiling(gib=4, reps=10):
+    """Practical HBM bandwidth of this GPU in GB/s: a 4 GiB FP64 read (sum), write
+    (fill_) and copy (copy_), each timed with events over `reps` calls after one."""
+    import torch
+    n = (gib << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+
+    def rate(f, nbytes):
+        f()
+        torch.cuda.synchronize()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
+            f()
+        t1.record()
+        torch.cuda.synchronize()
+        return nbytes / (t0.elapsed_time(t1) / reps * 1e-3) / 1e9
+
+    out = {"read": round(rate(lambda: a.sum(), n * 8), 1), "write": round(rate(lambda: b.fill_(2.0), n * 8), 1),
+           "copy": round(rate(lambda: b.copy_(a), 2 * n * 8), 1),
+           "method": f"{gib} GiB FP64 torch sum / fill_ / copy_, {reps} timed calls each"}
+    del a, b
+    torch.cuda.empty_cache()
+    return out
 
 
 def hbm_traffic(kernel_prefix, workload, ne, timeout_s=150):
@@ -258,11 +289,11 @@ def main():
         cfg_learned = atmos.AtmosConfig(prm, orientation=abi.FLAT_ORIENTATION,
                                         ref_profile=atmos.DryAdiabaticProfile(300.0, 0.0), gravity_source=True,
                                         turbulence=abi.LEARNED_CLOSURE, C_smag=0.23, numerical_flux=abi.RUSANOV)
-        tend_bytes, grad_bytes = TENDENCY_BYTES_MIXED, GRADIENT_BYTES_MIXED
+        tend_bytes, grad_bytes, stage_bytes = TENDENCY_BYTES_MIXED, GRADIENT_BYTES_MIXED, STAGE_BYTES_MIXED
     else:
         prm, grid, cfgs, aux, Q0, dt = build_tgv(ne)
         cfg_plain, cfg_learned = cfgs["smag"], cfgs["learned"]
-        tend_bytes, grad_bytes = TENDENCY_BYTES_MIXED_TGV, GRADIENT_BYTES_MIXED_TGV
+        tend_bytes, grad_bytes, stage_bytes = TENDENCY_BYTES_MIXED_TGV, GRADIENT_BYTES_MIXED_TGV, STAGE_BYTES_MIXED_TGV
     nodes = grid.Np * grid.nelem
     dof = nodes * 5
     print(f"[bench] {args.workload}: {grid.nelem} elements, {nodes} nodes, dt {dt:.4g} s "
@@ -319,6 +350,8 @@ def main():
         print(json.dumps({k: v for k, v in out.items()}), flush=True)
         return
 
+    ceiling = hbm_ceiling()
+    print(f"[bench] HBM ceiling on this GPU: {ceiling}", file=sys.stderr, flush=True)
     t_avg, g_avg = kt["tendency"][0], kt["gradient"][0]
     dom = ("k_tendency_mix (LSRK-fused, mixed)", t_avg, tend_bytes, "k_tendency_mix") if t_avg >= g_avg else \
           ("k_gradient_mix (learned closure fused)", g_avg, grad_bytes, "k_gradient_mix")
@@ -370,12 +403,21 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "measured_ceiling": ceiling,
+            "frac_of_measured_read_ceiling": round(achieved / ceiling["read"], 4),
             "traffic": None if traffic is None else int(traffic),
+            "traffic_rate_GBs": None if traffic is None else round(traffic / (dom[1] * 1e-3) / 1e9, 1),
             "traffic_detail": traffic_detail,
             "algorithmic_bytes_per_node": dom[2],
             "avg_launch_us": round(dom[1] * 1e3, 2),
             "k_tendency_avg_us": round(t_avg * 1e3, 2),
             "k_gradient_avg_us": round(g_avg * 1e3, 2),
+            "stage": {
+                "what": "algorithmic bytes of a whole RK stage (both kernels) over the stage time, ms_per_step / 5",
+                "bytes_per_node": stage_bytes,
+                "achieved": round(stage_bytes * nodes / (out["learned_ms"] / 5 * 1e-3) / 1e9, 1),
+                "frac": round(stage_bytes * nodes / (out["learned_ms"] / 5 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            },
         },
         "cpu_baseline": None,
     }
