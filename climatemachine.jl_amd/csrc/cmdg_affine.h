@@ -361,8 +361,13 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     constexpr int NGF = SMAG ? 10 : 9;
     __shared__ double sD[Nq * Nq];
+    // M(xi_a F) by direction; once every lane has its volume divergence, the same space
+    // holds the face fluxes (6 Nfp x 5) and the lanes' partial tendencies (5 x Np), so
+    // that the block needs 15 KB of LDS, not 21 KB (occupancy 4 waves per SIMD, not 3.5)
     __shared__ double sF[3][5][Np];
-    __shared__ double sFl[6 * Nfp][5];
+    static_assert(6 * Nfp * 5 + 5 * Np <= 3 * 5 * Np, "face fluxes and partials fit in sF");
+    double(*sFl)[5] = reinterpret_cast<double(*)[5]>(&sF[0][0][0]);
+    double(*sT)[Np] = reinterpret_cast<double(*)[Np]>(&sF[0][0][0] + 6 * Nfp * 5);
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     __shared__ int32_t nb[6];  // the element's face neighbours
@@ -415,16 +420,8 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
         }
     }
     __syncthreads();
-    // numerical fluxes of all face nodes, spread over every lane
-    for (int t = n; t < 6 * Nfp; t += NT) {
-        double fl5[5];
-        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr, fl5);
-#pragma unroll
-        for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
-    }
-    __syncthreads();
-
-    double qn[5];
+    // volume divergence and source of this node, into registers
+    double T[5];
     if (act) {
         double src[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         if (P.gravity) {
@@ -433,8 +430,7 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
 #pragma unroll
             for (int d = 0; d < 3; ++d) src[1 + d] = (-drho) * a.gPhi[d];
         }
-        double *tb = tend + (int64_t)Np * 5 * e + n;
-        double T[5];
+        const double *tb = tend + (int64_t)Np * 5 * e + n;
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
             double th = 0.0;
@@ -456,6 +452,24 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
             }
             T[s] = (MODE == 0 ? alpha * tv : tv) + out;
         }
+    }
+    __syncthreads();  // every read of sF is done: it now holds face fluxes and partials
+    if (act)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) sT[s][n] = T[s];
+    // numerical fluxes of all face nodes, spread over every lane
+    for (int t = n; t < 6 * Nfp; t += NT) {
+        double fl5[5];
+        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr, fl5);
+#pragma unroll
+        for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
+    }
+    __syncthreads();
+
+    double qn[5];
+    if (act) {
+#pragma unroll
+        for (int s = 0; s < 5; ++s) T[s] = sT[s][n];
         (void)u;
         (void)pm;
         // the face fluxes of this node's faces, in face order
@@ -468,6 +482,7 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
 #pragma unroll
             for (int s = 0; s < 5; ++s) T[s] -= w * sFl[f * Nfp + fn][s];
         }
+        double *tb = tend + (int64_t)Np * 5 * e + n;
         if (MODE == 0) {
 #pragma unroll
             for (int s = 0; s < 5; ++s) tb[Np * s] = T[s];
@@ -482,16 +497,16 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
         }
     }
     if (MODE == 1 && qtr_out) {
-        // the next stage's state trace, written face-major (contiguous per face)
-        __syncthreads();
+        // the next stage's state trace, written face-major (contiguous per face); the
+        // state is staged in this lane's own partial slots, which no other lane reads
         if (act)
 #pragma unroll
-            for (int s = 0; s < 5; ++s) sF[0][s][n] = qn[s];
+            for (int s = 0; s < 5; ++s) sT[s][n] = qn[s];
         __syncthreads();
         for (int t = n; t < 6 * Nfp; t += NT) {
             const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
 #pragma unroll
-            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sF[0][s][vn];
+            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sT[s][vn];
         }
     }
 }
