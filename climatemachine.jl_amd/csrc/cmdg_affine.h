@@ -290,7 +290,7 @@ template <int Nq, bool DIFF, bool SMAG>
 __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
                                               int32_t ne, const double *__restrict__ Q, const double *__restrict__ aux,
                                               const double *__restrict__ qtr, const double *__restrict__ f2tr,
-                                              double flux[5]) {
+                                              const double *f2own, double flux[5]) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
     const double nd = (f & 1) ? 1.0 : -1.0;
@@ -339,7 +339,7 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
         Fp[0] = 0.0;
 #pragma unroll
         for (int v = 0; v < TR_F2; ++v) {
-            Fm[1 + v] = f2tr[tri<Nfp>(e, f, v, TR_F2, fn)];
+            Fm[1 + v] = f2own[v];  // the same bits the gradient kernel put in the trace
             Fp[1 + v] = f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)];
         }
 #pragma unroll
@@ -368,6 +368,7 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
     static_assert(6 * Nfp * 5 + 5 * Np <= 3 * 5 * Np, "face fluxes and partials fit in sF");
     double(*sFl)[5] = reinterpret_cast<double(*)[5]>(&sF[0][0][0]);
     double(*sT)[Np] = reinterpret_cast<double(*)[Np]>(&sF[0][0][0] + 6 * Nfp * 5);
+    __shared__ double sF2o[DIFF ? 6 * Nfp : 1][TR_F2];  // this element's rows at its face nodes
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     __shared__ int32_t nb[6];  // the element's face neighbours
@@ -411,6 +412,11 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
             flux1_row(q, u, p, pm, ax, F);
             if (DIFF) {
                 flux2_row<SMAG>(P, q, gf, nu, ax, F2);
+                int fn;
+                const int f = node_face_dir<Nq>(i, j, k, ax, fn);
+                if (f >= 0)
+#pragma unroll
+                    for (int v = 0; v < TR_F2; ++v) sF2o[f * Nfp + fn][v] = F2[1 + v];
 #pragma unroll
                 for (int s = 0; s < 5; ++s) F[s] += F2[s];
             }
@@ -460,7 +466,8 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double fl5[5];
-        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr, fl5);
+        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr,
+                                      sF2o[DIFF ? t : 0], fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }

@@ -32,8 +32,12 @@ __device__ __forceinline__ void flux1_row_f(const float q[5], const float u[3], 
     F[4] = ud * (q[4] + p);
 }
 
+// strain_norm_f, viscosity_f and flux2_row_f round every operation (no contraction):
+// the gradient kernel evaluates the second-order flux rows of the face nodes for the
+// neighbours' trace and the tendency kernel evaluates the same rows for its own side,
+// and both must get the same bits for the face fluxes to stay conservative.
 __device__ __forceinline__ float strain_norm_f(const float *gf) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
     const float *S = gf + 3;
     const float norm2 = S[0] * S[0] + S[3] * S[3] + S[5] * S[5] + 2.0f * (S[1] * S[1] + S[2] * S[2] + S[4] * S[4]);
     return sqrtf(2.0f * norm2);
@@ -43,7 +47,7 @@ __device__ __forceinline__ float strain_norm_f(const float *gf) {
 template <bool SMAG, bool LEARN = false>
 __device__ __forceinline__ void viscosity_f(const MixParams &P, float rho, const float *gf, const float gPhi[3],
                                             float Delta, float nu[3]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
     if (SMAG) {
         const float normS = strain_norm_f(gf);
         const float ig = 1.0f / P.grav;
@@ -67,7 +71,7 @@ __device__ __forceinline__ void viscosity_f(const MixParams &P, float rho, const
 template <bool SMAG>
 __device__ __forceinline__ void flux2_row_f(const MixParams &P, const float q[5], const float *gf, const float nu[3],
                                             int d, float F[5]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
     const float *S = gf + 3;
     const float S0 = pick3(d, S[0], S[1], S[2]);
     const float S1 = pick3(d, S[1], S[3], S[4]);
@@ -304,7 +308,8 @@ template <int Nq, bool DIFF>
 __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
                                               int f, int fn, int32_t ne, const double *__restrict__ Q,
                                               const double *__restrict__ aux, const double *__restrict__ qtr,
-                                              const float *__restrict__ f2tr, double p_e, float flux[5]) {
+                                              const float *__restrict__ f2tr, const float *f2own, double p_e,
+                                              float flux[5]) {
 #pragma clang fp contract(fast)
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
@@ -362,7 +367,7 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     if (DIFF && !wall) {
 #pragma unroll
         for (int v = 0; v < TR_F2; ++v)
-            flux[1 + v] += (f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
+            flux[1 + v] += (f2own[v] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
     }
 }
 
@@ -379,6 +384,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     __shared__ float sD[Nq * Nq];
     __shared__ float sF[3][5][Np];
     __shared__ float sFl[6 * Nfp][5];
+    __shared__ float sF2o[DIFF ? 6 * Nfp : 1][TR_F2];  // this element's rows at its face nodes
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     __shared__ int32_t nb[6];  // the element's face neighbours
@@ -449,6 +455,11 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
             flux1_row_f(qf, uf, pf, pmf, ax, F);
             if (DIFF) {
                 flux2_row_f<SMAG>(MP, qf, gf, nu, ax, F2);
+                int fn;
+                const int f = node_face_dir<Nq>(i, j, k, ax, fn);
+                if (f >= 0)
+#pragma unroll
+                    for (int v = 0; v < TR_F2; ++v) sF2o[f * Nfp + fn][v] = F2[1 + v];
 #pragma unroll
                 for (int s = 0; s < 5; ++s) F[s] += F2[s];
             }
@@ -461,7 +472,8 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         float fl5[5];
-        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr, p_e, fl5);
+        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr,
+                                sF2o[DIFF ? t : 0], p_e, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }
