@@ -21,8 +21,11 @@ struct MixParams {
     int32_t dyn_visc, with_div, ref_sub, ref_src, gravity, rusanov;
 };
 
+// First-order flux row along axis d, every operation rounded (no contraction): the
+// tendency kernel evaluates an element's own row at a face node in its volume phase and
+// the neighbour evaluates the same row from the trace; both get the same bits.
 __device__ __forceinline__ void flux1_row_f(const float q[5], const float u[3], float p, float pm, int d, float F[5]) {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(off)
     const float qd = pick3(d, q[1], q[2], q[3]);
     const float ud = pick3(d, u[0], u[1], u[2]);
     F[0] = qd;
@@ -302,13 +305,17 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 }
 
 // ------------------------------------------------------------------ tendency (mixed)
-// Numerical flux at face node fn of face f of element e (FP32 fluxes from the FP64
-// face traces; the jump q- - q+ and the pressure perturbation in FP64).
+// Numerical flux at face node fn of face f of element e.  The own side arrives
+// precomputed by the volume phase (own[0..4]: the first-order flux row along the face's
+// axis, own[5..8]: the second-order row, own[9]: |u_d| + c); the neighbour's side is
+// evaluated here from its FP64 state trace (thermodynamics, p - p_ref in FP64), and
+// the jump q- - q+ is formed in FP64 from the own state (in cache from the volume
+// phase).
 template <int Nq, bool DIFF>
 __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
                                               int f, int fn, int32_t ne, const double *__restrict__ Q,
                                               const double *__restrict__ aux, const double *__restrict__ qtr,
-                                              const float *__restrict__ f2tr, const float *f2own, double p_e,
+                                              const float *__restrict__ f2tr, const float own[10], double p_e,
                                               float flux[5]) {
 #pragma clang fp contract(fast)
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
@@ -316,17 +323,16 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     const float h = (f & 1) ? 0.5f : -0.5f;
     const bool wall = ne < 0;
     const int fo = f ^ 1;
-    // the own side at the face's volume node (in cache from the volume phase), the
-    // neighbour's side from its trace
-    double q[5], qp[5];
+    double q[5], qp[5], Php, refpp;
 #pragma unroll
     for (int s = 0; s < 5; ++s) q[s] = Q[(int64_t)Np * (s + 5 * e) + vn];
-    const double Phi = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0), ref_p = own_aux<Nq>(P, aux, e, vn, P.a_ref_p, 0.0);
-    double Php = Phi, refpp = ref_p;
     if (!wall) {
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
-        if (!A.aux_cont) {
+        if (A.aux_cont) {
+            Php = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0);
+            refpp = own_aux<Nq>(P, aux, e, vn, P.a_ref_p, 0.0);
+        } else {
             Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
             refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
         }
@@ -336,38 +342,32 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             if (c == d) qp[1 + c] = -q[1 + c];  // reflect the normal momentum
+        Php = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0);
+        refpp = own_aux<Nq>(P, aux, e, vn, P.a_ref_p, 0.0);
     }
-    const ThermoD tm = thermo_d(P, q, Phi), tp = thermo_d(P, qp, Php);
-    float qf[5], qpf[5], uf[3], upf[3], jump[5];
+    const ThermoD tp = thermo_d(P, qp, Php);
+    float qpf[5], upf[3], jump[5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
-        qf[s] = (float)q[s];
         qpf[s] = (float)qp[s];
         jump[s] = (float)(q[s] - qp[s]);
     }
-    const float rim = 1.0f / qf[0], rip = 1.0f / qpf[0];
+    const float rip = 1.0f / qpf[0];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        uf[c] = qf[1 + c] * rim;
-        upf[c] = qpf[1 + c] * rip;
-    }
-    float Fm[5], Fp[5];
-    flux1_row_f(qf, uf, (float)tm.p, (float)(MP.ref_sub ? tm.p - ref_p : tm.p - p_e), d, Fm);
+    for (int c = 0; c < 3; ++c) upf[c] = qpf[1 + c] * rip;
+    float Fp[5];
     flux1_row_f(qpf, upf, (float)tp.p, (float)(MP.ref_sub ? tp.p - refpp : tp.p - p_e), d, Fp);
 #pragma unroll
-    for (int s = 0; s < 5; ++s) flux[s] = (Fm[s] + Fp[s]) * h;
+    for (int s = 0; s < 5; ++s) flux[s] = (own[s] + Fp[s]) * h;
     if (MP.rusanov) {
-        const float umd = pick3(d, uf[0], uf[1], uf[2]);
         const float upd = pick3(d, upf[0], upf[1], upf[2]);
-        const float mw = 0.5f * fmaxf(fabsf(umd) + sqrtf(MP.gammaR * (float)tm.temp),
-                                      fabsf(upd) + sqrtf(MP.gammaR * (float)tp.temp));
+        const float mw = 0.5f * fmaxf(own[9], fabsf(upd) + sqrtf(MP.gammaR * (float)tp.temp));
 #pragma unroll
         for (int s = 0; s < 5; ++s) flux[s] += mw * jump[s];
     }
     if (DIFF && !wall) {
 #pragma unroll
-        for (int v = 0; v < TR_F2; ++v)
-            flux[1 + v] += (f2own[v] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
+        for (int v = 0; v < TR_F2; ++v) flux[1 + v] += (own[5 + v] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
     }
 }
 
@@ -383,8 +383,9 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     static_assert(3 * 5 * sizeof(float) >= 5 * sizeof(double), "the state of the next stage is staged in sF");
     __shared__ float sD[Nq * Nq];
     __shared__ float sF[3][5][Np];
-    __shared__ float sFl[6 * Nfp][5];
-    __shared__ float sF2o[DIFF ? 6 * Nfp : 1][TR_F2];  // this element's rows at its face nodes
+    // per face node: the own side's flux rows and wave speed from the volume phase
+    // (first-order row 5, second-order row 4, |u_d| + c), then the face flux (first 5)
+    __shared__ float sFc[6 * Nfp][10];
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     __shared__ int32_t nb[6];  // the element's face neighbours
@@ -453,13 +454,18 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         for (int ax = 0; ax < 3; ++ax) {
             float F[5], F2[5];
             flux1_row_f(qf, uf, pf, pmf, ax, F);
+            int fn;
+            const int f = node_face_dir<Nq>(i, j, k, ax, fn);
+            if (f >= 0) {
+#pragma unroll
+                for (int s = 0; s < 5; ++s) sFc[f * Nfp + fn][s] = F[s];
+                sFc[f * Nfp + fn][9] = fabsf(uf[ax]) + cm;
+            }
             if (DIFF) {
                 flux2_row_f<SMAG>(MP, qf, gf, nu, ax, F2);
-                int fn;
-                const int f = node_face_dir<Nq>(i, j, k, ax, fn);
                 if (f >= 0)
 #pragma unroll
-                    for (int v = 0; v < TR_F2; ++v) sF2o[f * Nfp + fn][v] = F2[1 + v];
+                    for (int v = 0; v < TR_F2; ++v) sFc[f * Nfp + fn][5 + v] = F2[1 + v];
 #pragma unroll
                 for (int s = 0; s < 5; ++s) F[s] += F2[s];
             }
@@ -471,11 +477,12 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     __syncthreads();
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
-        float fl5[5];
-        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr,
-                                sF2o[DIFF ? t : 0], p_e, fl5);
+        float own[10], fl5[5];
 #pragma unroll
-        for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
+        for (int c = 0; c < 10; ++c) own[c] = sFc[t][c];
+        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr, own, p_e, fl5);
+#pragma unroll
+        for (int s = 0; s < 5; ++s) sFc[t][s] = fl5[s];
     }
     __syncthreads();
 
@@ -504,7 +511,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
             if (f < 0) continue;
             const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
 #pragma unroll
-            for (int s = 0; s < 5; ++s) T[s] -= w * sFl[f * Nfp + fn][s];
+            for (int s = 0; s < 5; ++s) T[s] -= w * sFc[f * Nfp + fn][s];
         }
         double *tb = tend + (int64_t)Np * 5 * e + n;
         if (MODE == 0) {
