@@ -372,3 +372,35 @@ def test_multistep_matches_single_steps(variant):
     assert np.array_equal(a, b)
     assert abs(sb.gettime() - 4 * dt) < 1e-15 and sb.steps == 4
     dg.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checkpoint_restart_is_bitwise(precision, tmp_path):
+    """Checkpoint after 3 of 6 steps (write_checkpoint: h_Q, h_aux, t), restart a new
+    context from the file (read_checkpoint; the reference's --restart-from-num path,
+    solver_configs.jl:129-160) and finish: the state equals the uninterrupted run's bit
+    for bit, and so do the auxiliary fields the kernels refresh."""
+    from cmdg import checkpoint as ck
+    from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
+    print(f"[parity] checkpoint restart {precision}", flush=True)
+    cfg, grid, aux, Q0 = _case("les_walls")
+    prec = abi.FP64 if precision == "fp64" else abi.MIXED
+    dt = 5e-3
+    dg = DGModel(cfg, grid, aux=aux, precision=prec)
+    Q = to_device(Q0)
+    s = LSRK54CarpenterKennedy(dg, Q, dt=dt, t0=0.0)
+    solve(Q, s, numberofsteps=3, timeend=math.inf)
+    ck.write_checkpoint(Q, dg.state_auxiliary, s.gettime(), str(tmp_path), "les walls", 1)
+    solve(Q, s, numberofsteps=3, timeend=math.inf)
+    ref, ref_aux = to_host(Q), to_host(dg.state_auxiliary)
+    dg.close()
+    hQ, haux, t = ck.read_checkpoint(str(tmp_path), "les walls", 1)
+    assert t == 3 * dt
+    dg2 = DGModel(cfg, grid, aux=haux, precision=prec)
+    Q2 = to_device(hQ)
+    s2 = LSRK54CarpenterKennedy(dg2, Q2, dt=dt, t0=t)
+    solve(Q2, s2, numberofsteps=3, timeend=math.inf)
+    assert np.isfinite(ref).all()
+    assert np.array_equal(to_host(Q2), ref)
+    assert np.array_equal(to_host(dg2.state_auxiliary), ref_aux)
+    dg2.close()
