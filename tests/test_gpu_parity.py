@@ -19,12 +19,12 @@ def rel_l2(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
 
 
-def _case(name):
+def _case(name, N=4):
     """(cfg, grid, aux, Q0) for a named parity case; inputs seeded and non-degenerate."""
-    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    rng = np.random.default_rng(zlib.crc32(name.encode()) + (0 if N == 4 else N))
     if name in ("euler_periodic", "viscous_periodic", "euler_central"):
         prm = atmos.ideal_gas_parameters()
-        grid = uniform_brick((3, 3, 2), (-5.0, -5.0, -5.0), (5.0, 5.0, 5.0), N=4)
+        grid = uniform_brick((3, 3, 2), (-5.0, -5.0, -5.0), (5.0, 5.0, 5.0), N=N)
         kw = dict(numerical_flux=abi.CENTRAL if name == "euler_central" else abi.RUSANOV)
         if name == "viscous_periodic":
             kw.update(turbulence=abi.CONSTANT_DYNAMIC_VISCOSITY, viscosity=5e-2, with_divergence=True)
@@ -39,7 +39,7 @@ def _case(name):
     if name == "tgv_kinematic":
         # Taylor-Green vortex, constant kinematic viscosity with the divergence term
         prm = atmos.ideal_gas_parameters()
-        grid = uniform_brick((3, 3, 3), (0.0, 0.0, 0.0), (2 * math.pi,) * 3, N=4)
+        grid = uniform_brick((3, 3, 3), (0.0, 0.0, 0.0), (2 * math.pi,) * 3, N=N)
         cfg = atmos.AtmosConfig(prm, turbulence=abi.CONSTANT_KINEMATIC_VISCOSITY, viscosity=0.02,
                                 with_divergence=True)
         aux = atmos.init_aux(cfg, grid)
@@ -48,7 +48,7 @@ def _case(name):
         return cfg, grid, aux, Q0
     # LES cases: walls in z, gravity, hydrostatic reference, Smagorinsky
     prm = atmos.ideal_gas_parameters()
-    grid = uniform_brick((3, 3, 3), (0.0, 0.0, 0.0), (300.0, 300.0, 300.0), N=4,
+    grid = uniform_brick((3, 3, 3), (0.0, 0.0, 0.0), (300.0, 300.0, 300.0), N=N,
                          periodicity=(True, True, False), boundary=((0, 0), (0, 0), (1, 2)))
     kw = dict(orientation=abi.FLAT_ORIENTATION, ref_profile=atmos.DryAdiabaticProfile(300.0, 0.0),
               gravity_source=True, turbulence=abi.SMAGORINSKY_LILLY, C_smag=0.23)
@@ -404,3 +404,51 @@ def test_checkpoint_restart_is_bitwise(precision, tmp_path):
     assert np.array_equal(to_host(Q2), ref)
     assert np.array_equal(to_host(dg2.state_auxiliary), ref_aux)
     dg2.close()
+
+
+@pytest.mark.parametrize("geometry", ["affine", "general"])
+@pytest.mark.parametrize("name", ["euler_periodic", "viscous_periodic", "les_walls"])
+def test_rhs_parity_order3(name, geometry):
+    """Polynomial order 3 (Nq = 4, one wave per element): the RHS against the oracle."""
+    import torch
+    from cmdg.dg import to_device, to_host
+    print(f"[parity] rhs N=3 {name} {geometry}", flush=True)
+    cfg, grid, aux, Q0 = _case(name, N=3)
+    assert grid.Nq == 4
+    orc = O.Oracle(cfg, grid, aux)
+    ref = orc.tendency(Q0)
+    dg = _model(cfg, grid, aux, geometry)
+    Q = to_device(Q0)
+    dQ = torch.full_like(Q, float("nan"))
+    dg(dQ, Q, None, 0.0)
+    got = to_host(dQ)
+    for s in range(5):
+        r = rel_l2(got[:, s], ref[:, s])
+        print(f"  state {s}: rel L2 {r:.3e}", flush=True)
+        assert r < RTOL, (name, s, r)
+    dg.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_lsrk54_order3(precision):
+    """Three fused LSRK54 steps at polynomial order 3: FP64 against the oracle to 1e-11,
+    the mixed path within its stated error."""
+    from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
+    print(f"[parity] lsrk54 N=3 les_walls {precision}", flush=True)
+    cfg, grid, aux, Q0 = _case("les_walls", N=3)
+    dt = 5e-3
+    orc = O.Oracle(cfg, grid, aux)
+    Qr = np.asfortranarray(Q0.copy())
+    dQr = np.zeros_like(Qr, order="F")
+    for k in range(3):
+        orc.lsrk54_step(Qr, dQr, k * dt, dt)
+    dg = DGModel(cfg, grid, aux=aux, precision=abi.FP64 if precision == "fp64" else abi.MIXED)
+    Q = to_device(Q0)
+    solve(Q, LSRK54CarpenterKennedy(dg, Q, dt=dt, t0=0.0), numberofsteps=3, timeend=math.inf)
+    got = to_host(Q)
+    tol = 1e-11 if precision == "fp64" else 2e-5
+    for s in range(5):
+        r = rel_l2(got[:, s] - Q0[:, s], Qr[:, s] - Q0[:, s])
+        print(f"  state {s}: rel L2 of the change {r:.3e}", flush=True)
+        assert r < tol, (precision, s, r)
+    dg.close()
