@@ -117,8 +117,8 @@ __device__ __forceinline__ double own_aux(const KParams &P, const double *__rest
 template <int Nq, bool SMAG>
 __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
                                                   int32_t ne, const double *__restrict__ Q,
-                                                  const double *__restrict__ qtr, const double *__restrict__ aux,
-                                                  const double (*sG)[Nq * Nq * Nq], double *dgf) {
+                                                  const double *__restrict__ aux, const double (*sG)[Nq * Nq * Nq],
+                                                  double *dgf) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
     const double nd = (f & 1) ? 1.0 : -1.0;
@@ -135,9 +135,12 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
     double Gp[5], Gs[5], nG[5];
     if (ne >= 0) {
         const int fo = f ^ 1;
+        // the neighbour's state at the shared node, from its volume (the gradient pass
+        // writes no state, so every block reads the stage's state)
         double qp[5];
+        const int64_t qb = (int64_t)Np * 5 * ne + face_node<Nq>(fo, fn);
 #pragma unroll
-        for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
+        for (int s = 0; s < 5; ++s) qp[s] = Q[qb + Np * s];
         const double Php = A.aux_cont ? own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0) : A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
         Thermo tp = thermo(P, qp, Php);
         grad_arg<SMAG>(P, qp, tp, SMAG ? theta_v(P, tp) : 0.0, Gp);
@@ -166,7 +169,7 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
 
 template <int Nq, bool SMAG>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__ Q, const double *__restrict__ qtr,
+k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__ Q, double *__restrict__ qtr_out,
                double *__restrict__ aux, double *__restrict__ gfo, double *__restrict__ f2tr, int write_aux) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     constexpr int NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
@@ -207,7 +210,7 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     // lifted central jumps of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double dgf[NGF];
-        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, qtr, aux, sG, dgf);
+        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, sG, dgf);
 #pragma unroll
         for (int s = 0; s < NGF; ++s) sJ[t][s] = dgf[s];
     }
@@ -274,11 +277,19 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
             for (int v = 0; v < TR_F2; ++v) sF2[d][v][n] = F2[1 + v];
         }
     }
+    // this stage's state, for the tendency pass's face trace: sG is no longer read
+    double(*sQ)[Np] = reinterpret_cast<double(*)[Np]>(&sG[0][0]);
+    if (qtr_out && act)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) sQ[s][n] = q[s];
     __syncthreads();
     for (int t = n; t < 6 * Nfp; t += NT) {
         const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
 #pragma unroll
         for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = sF2[f >> 1][v][vn];
+        if (qtr_out)
+#pragma unroll
+            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sQ[s][vn];
     }
 }
 
