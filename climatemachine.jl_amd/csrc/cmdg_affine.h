@@ -299,7 +299,7 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
 // walls bc_momentum.jl:24-49 / bc_energy.jl:10-17 (zero second-order flux).
 template <int Nq, bool DIFF, bool SMAG>
 __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
-                                              int32_t ne, const double *__restrict__ Q, const double *__restrict__ aux,
+                                              int32_t ne, const double (*sQ)[Nq * Nq * Nq],
                                               const double *__restrict__ qtr, const double *__restrict__ f2tr,
                                               const double *f2own, double flux[5]) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
@@ -308,12 +308,12 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
     const double h = nd * 0.5;
     const bool wall = ne < 0;
     const int fo = f ^ 1;
-    // the own side at the face's volume node: the block's volume phase has just read
-    // these lines, so they come from cache; the neighbour's side from its trace
+    // the own side at the face's volume node, staged in LDS (state rows 0-4, Phi 5,
+    // p_ref 6); the neighbour's side from its trace
     double q[5], qp[5];
 #pragma unroll
-    for (int s = 0; s < 5; ++s) q[s] = Q[(int64_t)Np * (s + 5 * e) + vn];
-    const double Phi = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0), ref_p = own_aux<Nq>(P, aux, e, vn, P.a_ref_p, 0.0);
+    for (int s = 0; s < 5; ++s) q[s] = sQ[s][vn];
+    const double Phi = sQ[5][vn], ref_p = sQ[6][vn];
     double Php = Phi, refpp = ref_p;
     if (!wall) {
 #pragma unroll
@@ -372,13 +372,13 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     constexpr int NGF = SMAG ? 10 : 9;
     __shared__ double sD[Nq * Nq];
-    // M(xi_a F) by direction; once every lane has its volume divergence, the same space
-    // holds the face fluxes (6 Nfp x 5) and the lanes' partial tendencies (5 x Np), so
-    // that the block needs 15 KB of LDS, not 21 KB (occupancy 4 waves per SIMD, not 3.5)
+    // M(xi_a F) by direction; once every lane has its volume divergence (in registers),
+    // the same space holds the face fluxes (6 Nfp x 5) and the element's state, Phi and
+    // p_ref for the face tasks (7 x Np), so that the block needs no more LDS
     __shared__ double sF[3][5][Np];
-    static_assert(6 * Nfp * 5 + 5 * Np <= 3 * 5 * Np, "face fluxes and partials fit in sF");
+    static_assert(6 * Nfp * 5 + 7 * Np <= 3 * 5 * Np, "face fluxes and staged state fit in sF");
     double(*sFl)[5] = reinterpret_cast<double(*)[5]>(&sF[0][0][0]);
-    double(*sT)[Np] = reinterpret_cast<double(*)[Np]>(&sF[0][0][0] + 6 * Nfp * 5);
+    double(*sQ)[Np] = reinterpret_cast<double(*)[Np]>(&sF[0][0][0] + 6 * Nfp * 5);
     __shared__ double sF2o[DIFF ? 6 * Nfp : 1][TR_F2];  // this element's rows at its face nodes
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
@@ -470,15 +470,19 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
             T[s] = (MODE == 0 ? alpha * tv : tv) + out;
         }
     }
-    __syncthreads();  // every read of sF is done: it now holds face fluxes and partials
-    if (act)
+    __syncthreads();  // every read of sF is done: it now holds the staged state, then face fluxes
+    if (act) {
 #pragma unroll
-        for (int s = 0; s < 5; ++s) sT[s][n] = T[s];
+        for (int s = 0; s < 5; ++s) sQ[s][n] = q[s];
+        sQ[5][n] = a.Phi;
+        sQ[6][n] = a.ref_p;
+    }
+    __syncthreads();
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double fl5[5];
-        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr,
-                                      sF2o[DIFF ? t : 0], fl5);
+        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], sQ, qtr, f2tr, sF2o[DIFF ? t : 0],
+                                      fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }
@@ -486,8 +490,6 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
 
     double qn[5];
     if (act) {
-#pragma unroll
-        for (int s = 0; s < 5; ++s) T[s] = sT[s][n];
         (void)u;
         (void)pm;
         // the face fluxes of this node's faces, in face order
@@ -515,16 +517,16 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
         }
     }
     if (MODE == 1 && qtr_out) {
-        // the next stage's state trace, written face-major (contiguous per face); the
-        // state is staged in this lane's own partial slots, which no other lane reads
+        // the next stage's state trace, written face-major (contiguous per face), staged
+        // where the face tasks read this stage's state (all of them are done)
         if (act)
 #pragma unroll
-            for (int s = 0; s < 5; ++s) sT[s][n] = qn[s];
+            for (int s = 0; s < 5; ++s) sQ[s][n] = qn[s];
         __syncthreads();
         for (int t = n; t < 6 * Nfp; t += NT) {
             const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
 #pragma unroll
-            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sT[s][vn];
+            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sQ[s][vn];
         }
     }
 }

@@ -331,14 +331,13 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 // precomputed by the volume phase (own[0..4]: the first-order flux row along the face's
 // axis, own[5..8]: the second-order row, own[9]: |u_d| + c); the neighbour's side is
 // evaluated here from its FP64 state trace (thermodynamics, p - p_ref in FP64), and
-// the jump q- - q+ is formed in FP64 from the own state (in cache from the volume
-// phase).
+// the jump q- - q+ is formed in FP64 from the own state, staged in LDS with Phi and
+// p_ref (sQ rows 0-4, 5, 6).
 template <int Nq, bool DIFF>
 __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
-                                              int f, int fn, int32_t ne, const double *__restrict__ Q,
-                                              const double *__restrict__ aux, const double *__restrict__ qtr,
-                                              const float *__restrict__ f2tr, const float own[10], double p_e,
-                                              float flux[5]) {
+                                              int f, int fn, int32_t ne, const double (*sQ)[Nq * Nq * Nq],
+                                              const double *__restrict__ qtr, const float *__restrict__ f2tr,
+                                              const float own[10], double p_e, float flux[5]) {
 #pragma clang fp contract(fast)
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
@@ -347,13 +346,13 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     const int fo = f ^ 1;
     double q[5], qp[5], Php, refpp;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) q[s] = Q[(int64_t)Np * (s + 5 * e) + vn];
+    for (int s = 0; s < 5; ++s) q[s] = sQ[s][vn];
     if (!wall) {
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
         if (A.aux_cont) {
-            Php = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0);
-            refpp = own_aux<Nq>(P, aux, e, vn, P.a_ref_p, 0.0);
+            Php = sQ[5][vn];
+            refpp = sQ[6][vn];
         } else {
             Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
             refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
@@ -364,8 +363,8 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
 #pragma unroll
         for (int c = 0; c < 3; ++c)
             if (c == d) qp[1 + c] = -q[1 + c];  // reflect the normal momentum
-        Php = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0);
-        refpp = own_aux<Nq>(P, aux, e, vn, P.a_ref_p, 0.0);
+        Php = sQ[5][vn];
+        refpp = sQ[6][vn];
     }
     const ThermoD tp = thermo_d(P, qp, Php);
     float qpf[5], upf[3], jump[5];
@@ -393,8 +392,9 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     }
 }
 
+// Six waves per SIMD: the register cap this asks for (80 at Nq = 5) spills nothing.
 template <int Nq, bool DIFF, bool SMAG, bool LEARN, int MODE>
-__global__ void __launch_bounds__(threads_for<Nq>())
+__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(6)))
 k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *Q,
                const double *__restrict__ qtr, const float *__restrict__ gfi, const float *__restrict__ f2tr,
                double *__restrict__ aux, double *__restrict__ tend, double alpha, double beta,
@@ -497,20 +497,9 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         }
     }
     __syncthreads();
-    // numerical fluxes of all face nodes, spread over every lane
-    for (int t = n; t < 6 * Nfp; t += NT) {
-        float own[10], fl5[5];
-#pragma unroll
-        for (int c = 0; c < 10; ++c) own[c] = sFc[t][c];
-        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, qtr, f2tr, own, p_e, fl5);
-#pragma unroll
-        for (int s = 0; s < 5; ++s) sFc[t][s] = fl5[s];
-    }
-    __syncthreads();
-
-    double qn[5];
+    // volume divergence and gravity source of this node, while sF holds the fluxes
+    float T[5];
     if (act) {
-        float T[5];
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
             float acc = 0.0f;
@@ -526,6 +515,32 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
 #pragma unroll
             for (int d = 0; d < 3; ++d) T[1 + d] -= drho * gPhi[d];
         }
+    }
+    __syncthreads();
+    // sF is free: it takes the element's state, Phi and p_ref (FP64) for the face tasks,
+    // which would otherwise read them again from memory after the cache has let them go
+    double(*sQ)[Np] = reinterpret_cast<double(*)[Np]>(&sF[0][0][0]);
+    static_assert(7 * Np * sizeof(double) <= sizeof(sF), "state, Phi and p_ref fit in sF");
+    if (act) {
+#pragma unroll
+        for (int s = 0; s < 5; ++s) sQ[s][n] = q[s];
+        sQ[5][n] = Phi;
+        sQ[6][n] = ref_p;
+    }
+    __syncthreads();
+    // numerical fluxes of all face nodes, spread over every lane
+    for (int t = n; t < 6 * Nfp; t += NT) {
+        float own[10], fl5[5];
+#pragma unroll
+        for (int c = 0; c < 10; ++c) own[c] = sFc[t][c];
+        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], sQ, qtr, f2tr, own, p_e, fl5);
+#pragma unroll
+        for (int s = 0; s < 5; ++s) sFc[t][s] = fl5[s];
+    }
+    __syncthreads();
+
+    double qn[5];
+    if (act) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             int fn;
@@ -553,7 +568,6 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     }
     if (MODE == 1 && qtr_out) {
         // the next stage's state trace, written face-major (contiguous per face)
-        double(*sQ)[Np] = reinterpret_cast<double(*)[Np]>(&sF[0][0][0]);
         __syncthreads();
         if (act)
 #pragma unroll
