@@ -157,7 +157,8 @@ template <int Nq> int launch_trace_impl(cmdg_ctx *c, const double *Q, double *qt
     return check_hip(c, hipGetLastError(), "trace launch");
 }
 
-// qtr_out (affine bricks): the face trace of Q this pass writes for the tendency pass.
+// qtr_out: the face trace of Q that the learned closure's gradient pass writes for the
+// tendency pass (null: none).
 template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *qtr_out, double *Qcopy, int write_aux) {
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
@@ -174,10 +175,10 @@ template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double 
                                c->nelem, Q, qtr_out, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
     } else if (c->affine) {
         if (c->smag)
-            hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q, qtr_out,
+            hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q,
                                c->d_aux, c->d_gf, c->d_f2tr, write_aux);
         else
-            hipLaunchKernelGGL((k_gradient_aff<Nq, false>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q, qtr_out,
+            hipLaunchKernelGGL((k_gradient_aff<Nq, false>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q,
                                c->d_aux, c->d_gf, c->d_f2tr, write_aux);
     } else if (c->smag)
         hipLaunchKernelGGL((k_gradient<Nq, true>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_aux, c->d_gf, Qcopy, write_aux);
@@ -255,11 +256,11 @@ int launch_tendency(cmdg_ctx *c, const double *Q, const double *qtr, double *ten
 int launch_rhs(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta) {
     int rc;
     if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
-    // affine bricks: the face trace of Q comes from the gradient pass, or from the
-    // trace kernel when there is none
+    // affine bricks: the face trace of Q comes from the learned closure's gradient pass,
+    // else from the trace kernel
     double *qtr = c->affine ? c->d_qtr[0] : nullptr;
-    if (c->affine && !c->diffusive && (rc = launch_trace(c, Q, qtr, nullptr))) return rc;
-    if (c->diffusive && (rc = launch_gradient(c, Q, qtr, nullptr, 1))) return rc;
+    if (c->affine && !c->learned && (rc = launch_trace(c, Q, qtr, nullptr))) return rc;
+    if (c->diffusive && (rc = launch_gradient(c, Q, c->learned ? qtr : nullptr, nullptr, 1))) return rc;
     return launch_tendency(c, Q, qtr, tend, alpha, beta, 0, nullptr, nullptr, 0.0, 0.0, c->diffusive ? 0 : 1);
 }
 

@@ -169,8 +169,8 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
 
 template <int Nq, bool SMAG>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__ Q, double *__restrict__ qtr_out,
-               double *__restrict__ aux, double *__restrict__ gfo, double *__restrict__ f2tr, int write_aux) {
+k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__ Q, double *__restrict__ aux,
+               double *__restrict__ gfo, double *__restrict__ f2tr, int write_aux) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     constexpr int NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
     __shared__ double sD[Nq * Nq];
@@ -277,19 +277,11 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
             for (int v = 0; v < TR_F2; ++v) sF2[d][v][n] = F2[1 + v];
         }
     }
-    // this stage's state, for the tendency pass's face trace: sG is no longer read
-    double(*sQ)[Np] = reinterpret_cast<double(*)[Np]>(&sG[0][0]);
-    if (qtr_out && act)
-#pragma unroll
-        for (int s = 0; s < 5; ++s) sQ[s][n] = q[s];
     __syncthreads();
     for (int t = n; t < 6 * Nfp; t += NT) {
         const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
 #pragma unroll
         for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = sF2[f >> 1][v][vn];
-        if (qtr_out)
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sQ[s][vn];
     }
 }
 
