@@ -157,22 +157,20 @@ template <int Nq> int launch_trace_impl(cmdg_ctx *c, const double *Q, double *qt
     return check_hip(c, hipGetLastError(), "trace launch");
 }
 
-// qtr_out: the face trace of Q that the learned closure's gradient pass writes for the
-// tendency pass (null: none).
-template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *qtr_out, double *Qcopy, int write_aux) {
+template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux) {
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_GRADIENT);
     if (c->mixed) {
         if (c->learned)
             hipLaunchKernelGGL((k_gradient_mix<Nq, true, true>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem,
-                               Q, qtr_out, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
+                               Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
         else if (c->smag)
             hipLaunchKernelGGL((k_gradient_mix<Nq, true, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem,
-                               Q, qtr_out, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
+                               Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
         else
             hipLaunchKernelGGL((k_gradient_mix<Nq, false, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA,
-                               c->nelem, Q, qtr_out, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
+                               c->nelem, Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
     } else if (c->affine) {
         if (c->smag)
             hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q,
@@ -239,8 +237,8 @@ int launch_trace(cmdg_ctx *c, const double *Q, double *qtr, double *Qcopy) {
 #undef CALL
 }
 
-int launch_gradient(cmdg_ctx *c, const double *Q, double *qtr_out, double *Qcopy, int write_aux) {
-#define CALL(NQ) launch_gradient_impl<NQ>(c, Q, qtr_out, Qcopy, write_aux)
+int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux) {
+#define CALL(NQ) launch_gradient_impl<NQ>(c, Q, Qcopy, write_aux)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
 }
@@ -256,11 +254,9 @@ int launch_tendency(cmdg_ctx *c, const double *Q, const double *qtr, double *ten
 int launch_rhs(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta) {
     int rc;
     if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
-    // affine bricks: the face trace of Q comes from the learned closure's gradient pass,
-    // else from the trace kernel
-    double *qtr = c->affine ? c->d_qtr[0] : nullptr;
-    if (c->affine && !c->learned && (rc = launch_trace(c, Q, qtr, nullptr))) return rc;
-    if (c->diffusive && (rc = launch_gradient(c, Q, c->learned ? qtr : nullptr, nullptr, 1))) return rc;
+    double *qtr = c->affine ? c->d_qtr[0] : nullptr;  // the face trace of Q (affine bricks)
+    if (c->affine && (rc = launch_trace(c, Q, qtr, nullptr))) return rc;
+    if (c->diffusive && (rc = launch_gradient(c, Q, nullptr, 1))) return rc;
     return launch_tendency(c, Q, qtr, tend, alpha, beta, 0, nullptr, nullptr, 0.0, 0.0, c->diffusive ? 0 : 1);
 }
 
@@ -627,26 +623,13 @@ int cmdg_lsrk54_steps(cmdg_ctx *c, double *Q, double *dQ, double t, double dt, i
     if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
     if (nsteps == 0) return CMDG_OK;
     int rc;
-    if (c->affine && c->learned) {
+    if (c->affine) {
         // Affine bricks: the tendency pass reads its neighbours only through the state's
         // face trace, never their volume state, so every stage updates Q in place; the
-        // gradient pass, which writes no state, reads the neighbours' volume state.
-        // With the learned closure the gradient pass also writes the stage's face trace
-        // (its network leaves the memory system idle, so the writes come nearly free);
-        // the other paths write it in the tendency pass's epilogue for the next stage.
-        for (int64_t step = 0; step < nsteps; ++step)
-            for (int s = 0; s < 5; ++s) {
-                const double a = RKA[(s + 1) % 5], bdt = RKB[s] * dt;
-                const int wa = (s == 4);  // aux keeps the state of the last stage, as after dostep!
-                if ((rc = launch_gradient(c, Q, c->d_qtr[0], nullptr, wa))) return rc;
-                if ((rc = launch_tendency(c, Q, c->d_qtr[0], dQ, 1.0, 1.0, 1, Q, nullptr, a, bdt, 0))) return rc;
-            }
-        return CMDG_OK;
-    }
-    if (c->affine) {
-        // The trace is ping-ponged: written from Q by the trace kernel once, then by each
-        // stage's epilogue for the next stage -- across step boundaries too, since no
-        // caller code runs between the steps of one call.
+        // gradient pass, which writes no state, gathers the neighbours' face values from
+        // their volume state.  The trace is ping-ponged: written from Q by the trace
+        // kernel once, then by each stage's epilogue for the next stage -- across step
+        // boundaries too, since no caller code runs between the steps of one call.
         if ((rc = launch_trace(c, Q, c->d_qtr[0], nullptr))) return rc;
         int cur = 0;
         for (int64_t step = 0; step < nsteps; ++step)
@@ -655,7 +638,7 @@ int cmdg_lsrk54_steps(cmdg_ctx *c, double *Q, double *dQ, double t, double dt, i
                 const int wa = (s == 4);  // aux keeps the state of the last stage, as after dostep!
                 const bool last = (step == nsteps - 1) && s == 4;
                 double *qtr = c->d_qtr[cur], *qtr_out = last ? nullptr : c->d_qtr[cur ^ 1];
-                if (c->diffusive && (rc = launch_gradient(c, Q, nullptr, nullptr, wa))) return rc;
+                if (c->diffusive && (rc = launch_gradient(c, Q, nullptr, wa))) return rc;
                 if ((rc = launch_tendency(c, Q, qtr, dQ, 1.0, 1.0, 1, Q, qtr_out, a, bdt, c->diffusive ? 0 : wa)))
                     return rc;
                 cur ^= 1;
@@ -677,7 +660,7 @@ int cmdg_lsrk54_steps(cmdg_ctx *c, double *Q, double *dQ, double t, double dt, i
             if (c->diffusive) {
                 // the first gradient pass also leaves S <- C
                 double *Qcopy = (s == 0) ? S : nullptr;
-                if ((rc = launch_gradient(c, (s == 0) ? C : src, nullptr, Qcopy, wa))) return rc;
+                if ((rc = launch_gradient(c, (s == 0) ? C : src, Qcopy, wa))) return rc;
             }
             if ((rc = launch_tendency(c, src, nullptr, dQ, 1.0, 1.0, 1, dst, nullptr, a, bdt, c->diffusive ? 0 : wa)))
                 return rc;

@@ -148,7 +148,7 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 template <int Nq, bool SMAG, bool LEARN>
 __global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(5)))
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
-               double *__restrict__ qtr_out, double *__restrict__ aux, float *__restrict__ gfo,
+               double *__restrict__ aux, float *__restrict__ gfo,
                float *__restrict__ f2tr, int write_aux, const MlpDev *__restrict__ mlp) {
 #pragma clang fp contract(fast)
     static_assert(!LEARN || SMAG, "the learned closure keeps the Smagorinsky-Lilly structure");
@@ -156,12 +156,8 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     constexpr int NG = SMAG ? 5 : 4, NGF = LEARN ? 11 : (SMAG ? 10 : 9);
     constexpr int NT = threads_for<Nq>();
     __shared__ float sD[Nq * Nq];
-    // G by node, then the lifted jumps by face node; at the end the same space stages
-    // this stage's state (FP64) for the face trace
-    __shared__ __attribute__((aligned(16))) float sGJ[NG * Np + 6 * Nfp * 5];
-    static_assert(5 * Np * sizeof(double) <= sizeof(sGJ), "the state fits where G and the jumps were");
-    float(*sG)[Np] = reinterpret_cast<float(*)[Np]>(sGJ);
-    float(*sJ)[5] = reinterpret_cast<float(*)[5]>(sGJ + NG * Np);
+    __shared__ float sG[NG][Np];
+    __shared__ float sJ[6 * Nfp][5];
     __shared__ float sF2[3][TR_F2][Np];
     __shared__ uint16_t sX[LEARN ? (NT / 64) * MLP_LDS_PER_WAVE : 1];
     const int64_t e = xcd_element(blockIdx.x, nelem);
@@ -303,26 +299,11 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
         }
     }
     __syncthreads();
-    // ... written to the trace face-major; G and the jumps are no longer read, so their
-    // space takes this stage's state for the tendency pass's face trace
-    double(*sQ)[Np] = reinterpret_cast<double(*)[Np]>(sGJ);
-    if (qtr_out && act) {
-        const double *qb = Q + (int64_t)Np * 5 * e + n;  // read again: cheaper than keeping it in registers
-#pragma unroll
-        for (int s = 0; s < 5; ++s) sQ[s][n] = qb[Np * s];
-    }
+    // ... written to the trace face-major
     for (int t = n; t < 6 * Nfp; t += NT) {
         const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
 #pragma unroll
         for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = sF2[f >> 1][v][vn];
-    }
-    if (qtr_out) {
-        __syncthreads();
-        for (int t = n; t < 6 * Nfp; t += NT) {
-            const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sQ[s][vn];
-        }
     }
 }
 
