@@ -150,10 +150,10 @@ template <class T> int upload(cmdg_ctx *c, T **dst, const T *src, size_t count) 
     return CMDG_OK;
 }
 
-template <int Nq> int launch_trace_impl(cmdg_ctx *c, const double *Q, double *qtr, double *Qcopy) {
+template <int Nq> int launch_trace_impl(cmdg_ctx *c, const double *Q, double *qtr) {
     dim3 grid((unsigned)xcd_grid(c->nelem)), block(threads_for<Nq>());
     TimedLaunch tl(c, KIND_COPY);
-    hipLaunchKernelGGL((k_trace_aff<Nq>), grid, block, 0, c->stream, c->nelem, Q, qtr, Qcopy);
+    hipLaunchKernelGGL((k_trace_aff<Nq>), grid, block, 0, c->stream, c->nelem, Q, qtr);
     return check_hip(c, hipGetLastError(), "trace launch");
 }
 
@@ -231,8 +231,8 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double
     default: return fail(c, CMDG_EUNSUPPORTED, "polynomial order not compiled"); \
     }
 
-int launch_trace(cmdg_ctx *c, const double *Q, double *qtr, double *Qcopy) {
-#define CALL(NQ) launch_trace_impl<NQ>(c, Q, qtr, Qcopy)
+int launch_trace(cmdg_ctx *c, const double *Q, double *qtr) {
+#define CALL(NQ) launch_trace_impl<NQ>(c, Q, qtr)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
 }
@@ -255,7 +255,7 @@ int launch_rhs(cmdg_ctx *c, const double *Q, double *tend, double alpha, double 
     int rc;
     if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
     double *qtr = c->affine ? c->d_qtr[0] : nullptr;  // the face trace of Q (affine bricks)
-    if (c->affine && (rc = launch_trace(c, Q, qtr, nullptr))) return rc;
+    if (c->affine && (rc = launch_trace(c, Q, qtr))) return rc;
     if (c->diffusive && (rc = launch_gradient(c, Q, nullptr, 1))) return rc;
     return launch_tendency(c, Q, qtr, tend, alpha, beta, 0, nullptr, nullptr, 0.0, 0.0, c->diffusive ? 0 : 1);
 }
@@ -630,7 +630,7 @@ int cmdg_lsrk54_steps(cmdg_ctx *c, double *Q, double *dQ, double t, double dt, i
         // their volume state.  The trace is ping-ponged: written from Q by the trace
         // kernel once, then by each stage's epilogue for the next stage -- across step
         // boundaries too, since no caller code runs between the steps of one call.
-        if ((rc = launch_trace(c, Q, c->d_qtr[0], nullptr))) return rc;
+        if ((rc = launch_trace(c, Q, c->d_qtr[0]))) return rc;
         int cur = 0;
         for (int64_t step = 0; step < nsteps; ++step)
             for (int s = 0; s < 5; ++s) {
@@ -738,7 +738,7 @@ int cmdg_velocity_gradient(cmdg_ctx *c, const double *Q, double *grad_u) {
     if (!c) return CMDG_EINVAL;
     if (!Q || !grad_u) return fail(c, CMDG_EINVAL, "null pointer");
     if (!c->affine) return fail(c, CMDG_EUNSUPPORTED, "the velocity gradient is built for affine brick meshes");
-    int rc = launch_trace(c, Q, c->d_qtr[0], nullptr);
+    int rc = launch_trace(c, Q, c->d_qtr[0]);
     if (rc) return rc;
     dim3 grid((unsigned)xcd_grid(c->nelem));
     if (c->Nq == 5)

@@ -78,20 +78,15 @@ template <int Nq> __device__ __forceinline__ int face_node(int f, int fn) {
 // the face traces.
 
 // ------------------------------------------------------------------ state trace
-// Writes the face-node trace of Q (and optionally a copy of Q).
+// Writes the face-node trace of Q.
 template <int Nq>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_trace_aff(int64_t nelem, const double *__restrict__ Q, double *__restrict__ qtr, double *__restrict__ Qcopy) {
+k_trace_aff(int64_t nelem, const double *__restrict__ Q, double *__restrict__ qtr) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     const int64_t e = xcd_element(blockIdx.x, nelem);
     if (e >= nelem) return;
     const int n = threadIdx.x;
     const double *qb = Q + (int64_t)Np * 5 * e;
-    if (Qcopy && n < Np) {
-        double *qc = Qcopy + (int64_t)Np * 5 * e + n;
-#pragma unroll
-        for (int s = 0; s < 5; ++s) qc[Np * s] = qb[Np * s + n];
-    }
     // face-major: consecutive lanes write consecutive face nodes
     for (int t = n; t < 6 * Nfp; t += NT) {
         const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
