@@ -39,6 +39,19 @@ struct AffArrays {
     const float *gphi32;  // (Np, 3, nelem) grad Phi rounded to FP32 (mixed path; null without it)
 };
 
+// The elements one launch works on: the range first .. first + count - 1, or, with a
+// list, list[0 .. count - 1].  Block b takes the element at position xcd_element(b, count):
+// each XCD gets one contiguous run of the set.
+struct ElemSet {
+    const int32_t *list;
+    int64_t first, count;
+};
+__device__ __forceinline__ int64_t set_element(const ElemSet &S, int64_t b) {
+    const int64_t p = xcd_element(b, S.count);
+    if (p >= S.count) return -1;
+    return S.list ? (int64_t)S.list[p] : S.first + p;
+}
+
 template <int Nfp> __device__ __forceinline__ int64_t tri(int64_t e, int f, int v, int nv, int fn) {
     return ((e * 6 + f) * nv + v) * (int64_t)Nfp + fn;
 }
@@ -164,7 +177,7 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
 
 template <int Nq, bool SMAG>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__ Q, double *__restrict__ aux,
+k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q, double *__restrict__ aux,
                double *__restrict__ gfo, double *__restrict__ f2tr, int write_aux) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     constexpr int NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
@@ -174,8 +187,8 @@ k_gradient_aff(KParams P, AffArrays A, int64_t nelem, const double *__restrict__
     constexpr int NJ = 6 * Nfp * NGF > 3 * TR_F2 * Np ? 6 * Nfp * NGF : 3 * TR_F2 * Np;
     __shared__ double sJbuf[NJ];
     double(*sJ)[NGF] = reinterpret_cast<double(*)[NGF]>(sJbuf);
-    const int64_t e = xcd_element(blockIdx.x, nelem);
-    if (e >= nelem) return;
+    const int64_t e = set_element(ES, blockIdx.x);
+    if (e < 0) return;
     __shared__ int32_t nb[6];  // the element's face neighbours
     if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
@@ -352,7 +365,7 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
 //   precede the workgroup barrier that comes before its writes.
 template <int Nq, bool DIFF, bool SMAG, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const double *__restrict__ qtr,
+k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *Q, const double *__restrict__ qtr,
                const double *__restrict__ gfi, const double *__restrict__ f2tr, double *__restrict__ aux,
                double *__restrict__ tend, double alpha, double beta, double *Qout,
                double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
@@ -367,8 +380,8 @@ k_tendency_aff(KParams P, AffArrays A, int64_t nelem, const double *Q, const dou
     double(*sFl)[5] = reinterpret_cast<double(*)[5]>(&sF[0][0][0]);
     double(*sQ)[Np] = reinterpret_cast<double(*)[Np]>(&sF[0][0][0] + 6 * Nfp * 5);
     __shared__ double sF2o[DIFF ? 6 * Nfp : 1][TR_F2];  // this element's rows at its face nodes
-    const int64_t e = xcd_element(blockIdx.x, nelem);
-    if (e >= nelem) return;
+    const int64_t e = set_element(ES, blockIdx.x);
+    if (e < 0) return;
     __shared__ int32_t nb[6];  // the element's face neighbours
     if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;

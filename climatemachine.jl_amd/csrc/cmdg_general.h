@@ -89,11 +89,12 @@ k_gradient(KParams P, KArrays A, const double *__restrict__ Q, double *__restric
     // lifted face jumps: M^-1 Mf (Phi(G*) - Phi(n G-)), CentralNumericalFluxGradient
     double Gm[5];
     grad_arg<SMAG>(P, q, t, thv, Gm);
-    int fl[3], fnl[3];
-    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
-    for (int c = 0; c < nf; ++c) {
-        const int f = fl[c];
-        const int64_t idx = fnl[c] + (int64_t)Nfp * (f + 6 * e);
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        int fnd;
+        const int f = node_face_dir<Nq>(i, j, k, dir, fnd);
+        if (f < 0) continue;
+        const int64_t idx = fnd + (int64_t)Nfp * (f + 6 * e);
         const double *sgp = A.sg + 5 * idx;
         double nv[3] = {sgp[0], sgp[1], sgp[2]};
         const double sM = sgp[3], vMI = sgp[4];
@@ -230,13 +231,15 @@ k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__r
     }
 
     // face fluxes of the faces this node lies on (NumericalFluxes.jl:219-340, 670-715)
-    int fl[3], fnl[3];
-    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
+    const bool on_face = i == 0 || i == Nq - 1 || j == 0 || j == Nq - 1 || k == 0 || k == Nq - 1;
     double csm = 0.0;
-    if (nf > 0 && P.rusanov) csm = sqrt(P.gammaR * t.temp);
-    for (int c = 0; c < nf; ++c) {
-        const int f = fl[c];
-        const int64_t idx = fnl[c] + (int64_t)Nfp * (f + 6 * e);
+    if (on_face && P.rusanov) csm = sqrt(P.gammaR * t.temp);
+#pragma unroll
+    for (int dir = 0; dir < 3; ++dir) {
+        int fnd;
+        const int f = node_face_dir<Nq>(i, j, k, dir, fnd);
+        if (f < 0) continue;
+        const int64_t idx = fnd + (int64_t)Nfp * (f + 6 * e);
         const double *sgp = A.sg + 5 * idx;
         const double nv[3] = {sgp[0], sgp[1], sgp[2]};
         const double w = (MODE == 0 ? alpha * sgp[4] : sgp[4]) * sgp[3];  // alpha * vMI * sM

@@ -51,10 +51,11 @@ k_velgrad_aff(AffArrays A, int64_t nelem, const double *__restrict__ Q, const do
         g[2][c] = ge[2] * g3;
     }
     const double MI = 1.0 / (ge[3] * A.W[n]);
-    int fl[3], fnl[3];
-    const int nf = node_faces<Nq>(i, j, k, fl, fnl);
-    for (int c = 0; c < nf; ++c) {
-        const int f = fl[c], d = f >> 1, fn = fnl[c];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        int fn;
+        const int f = node_face_dir<Nq>(i, j, k, d, fn);
+        if (f < 0) continue;
         const double w = MI * (ge[4 + f] * A.Wf[fn]) * ((f & 1) ? 1.0 : -1.0);
         const int32_t ne = A.nbe[6 * e + f];
         double jump[3];

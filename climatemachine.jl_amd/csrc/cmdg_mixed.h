@@ -147,7 +147,7 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 // register cap this asks for (96) spills nothing at Nq = 5.
 template <int Nq, bool SMAG, bool LEARN>
 __global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(5)))
-k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *__restrict__ Q,
+k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
                double *__restrict__ aux, float *__restrict__ gfo,
                float *__restrict__ f2tr, int write_aux, const MlpDev *__restrict__ mlp) {
 #pragma clang fp contract(fast)
@@ -160,8 +160,8 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     __shared__ float sJ[6 * Nfp][5];
     __shared__ float sF2[3][TR_F2][Np];
     __shared__ uint16_t sX[LEARN ? (NT / 64) * MLP_LDS_PER_WAVE : 1];
-    const int64_t e = xcd_element(blockIdx.x, nelem);
-    if (e >= nelem) return;
+    const int64_t e = set_element(ES, blockIdx.x);
+    if (e < 0) return;
     __shared__ int32_t nb[6];  // the element's face neighbours
     if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
@@ -373,10 +373,11 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     }
 }
 
-// Six waves per SIMD: the register cap this asks for (80 at Nq = 5) spills nothing.
+// Six waves per SIMD at Nq = 5: the register cap this asks for (80) spills nothing there
+// (at Nq = 4 it would, so that order keeps the compiler's choice).
 template <int Nq, bool DIFF, bool SMAG, bool LEARN, int MODE>
-__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(6)))
-k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double *Q,
+__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(Nq == 5 ? 6 : 1)))
+k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *Q,
                const double *__restrict__ qtr, const float *__restrict__ gfi, const float *__restrict__ f2tr,
                double *__restrict__ aux, double *__restrict__ tend, double alpha, double beta,
                double *Qout, double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
@@ -389,8 +390,8 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, int64_t nelem, const double
     // per face node: the own side's flux rows and wave speed from the volume phase
     // (first-order row 5, second-order row 4, |u_d| + c), then the face flux (first 5)
     __shared__ float sFc[6 * Nfp][10];
-    const int64_t e = xcd_element(blockIdx.x, nelem);
-    if (e >= nelem) return;
+    const int64_t e = set_element(ES, blockIdx.x);
+    if (e < 0) return;
     __shared__ int32_t nb[6];  // the element's face neighbours
     if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
