@@ -157,26 +157,28 @@ template <int Nq> int launch_trace_impl(cmdg_ctx *c, const double *Q, double *qt
     return check_hip(c, hipGetLastError(), "trace launch");
 }
 
-template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux) {
+// ES: the elements of this launch (affine kernels; the general ones take them all).
+template <int Nq>
+int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, const ElemSet &ES) {
     constexpr int NT = threads_for<Nq>();
-    dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
+    dim3 grid((unsigned)xcd_grid(c->affine ? ES.count : c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_GRADIENT);
     if (c->mixed) {
         if (c->learned)
-            hipLaunchKernelGGL((k_gradient_mix<Nq, true, true>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem,
+            hipLaunchKernelGGL((k_gradient_mix<Nq, true, true>), grid, block, 0, c->stream, c->P, c->MP, c->AA, ES,
                                Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
         else if (c->smag)
-            hipLaunchKernelGGL((k_gradient_mix<Nq, true, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA, c->nelem,
+            hipLaunchKernelGGL((k_gradient_mix<Nq, true, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA, ES,
                                Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
         else
             hipLaunchKernelGGL((k_gradient_mix<Nq, false, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA,
-                               c->nelem, Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
+                               ES, Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
     } else if (c->affine) {
         if (c->smag)
-            hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q,
+            hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, ES, Q,
                                c->d_aux, c->d_gf, c->d_f2tr, write_aux);
         else
-            hipLaunchKernelGGL((k_gradient_aff<Nq, false>), grid, block, 0, c->stream, c->P, c->AA, c->nelem, Q,
+            hipLaunchKernelGGL((k_gradient_aff<Nq, false>), grid, block, 0, c->stream, c->P, c->AA, ES, Q,
                                c->d_aux, c->d_gf, c->d_f2tr, write_aux);
     } else if (c->smag)
         hipLaunchKernelGGL((k_gradient<Nq, true>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_aux, c->d_gf, Qcopy, write_aux);
@@ -187,13 +189,14 @@ template <int Nq> int launch_gradient_impl(cmdg_ctx *c, const double *Q, double 
 
 template <int Nq>
 int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double *tend, double alpha, double beta,
-                         int mode, double *Qout, double *qtr_out, double rka, double bdt, int write_aux) {
+                         int mode, double *Qout, double *qtr_out, double rka, double bdt, int write_aux,
+                         const ElemSet &ES) {
     constexpr int NT = threads_for<Nq>();
-    dim3 grid((unsigned)xcd_grid(c->nelem)), block(NT);
+    dim3 grid((unsigned)xcd_grid(c->affine ? ES.count : c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_TENDENCY);
 #define CMDG_TEND_MIX(DIFF, SMAG, LEARN, MODE)                                                                   \
     hipLaunchKernelGGL((k_tendency_mix<Nq, DIFF, SMAG, LEARN, MODE>), grid, block, 0, c->stream, c->P, c->MP, c->AA, \
-                       c->nelem, Q, qtr, c->d_gf32, c->d_f2tr32, c->d_aux, tend, alpha, beta, Qout, qtr_out, rka,   \
+                       ES, Q, qtr, c->d_gf32, c->d_f2tr32, c->d_aux, tend, alpha, beta, Qout, qtr_out, rka,   \
                        bdt, write_aux)
 #define CMDG_TEND(DIFF, SMAG, MODE)                                                                               \
     do {                                                                                                          \
@@ -201,7 +204,7 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double
             CMDG_TEND_MIX(DIFF, SMAG, false, MODE);                                                               \
         else if (c->affine)                                                                                       \
             hipLaunchKernelGGL((k_tendency_aff<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->AA,    \
-                               c->nelem, Q, qtr, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, qtr_out,  \
+                               ES, Q, qtr, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, qtr_out,  \
                                rka, bdt, write_aux);                                                              \
         else                                                                                                      \
             hipLaunchKernelGGL((k_tendency<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->A, Q,      \
@@ -237,15 +240,19 @@ int launch_trace(cmdg_ctx *c, const double *Q, double *qtr) {
 #undef CALL
 }
 
-int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux) {
-#define CALL(NQ) launch_gradient_impl<NQ>(c, Q, Qcopy, write_aux)
+int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, const ElemSet *ES = nullptr) {
+    const ElemSet all{nullptr, 0, c->nelem};
+#define CALL(NQ) launch_gradient_impl<NQ>(c, Q, Qcopy, write_aux, ES ? *ES : all)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
 }
 
 int launch_tendency(cmdg_ctx *c, const double *Q, const double *qtr, double *tend, double alpha, double beta, int mode,
-                    double *Qout, double *qtr_out, double rka, double bdt, int write_aux) {
-#define CALL(NQ) launch_tendency_impl<NQ>(c, Q, qtr, tend, alpha, beta, mode, Qout, qtr_out, rka, bdt, write_aux)
+                    double *Qout, double *qtr_out, double rka, double bdt, int write_aux,
+                    const ElemSet *ES = nullptr) {
+    const ElemSet all{nullptr, 0, c->nelem};
+#define CALL(NQ) launch_tendency_impl<NQ>(c, Q, qtr, tend, alpha, beta, mode, Qout, qtr_out, rka, bdt, write_aux, \
+                                          ES ? *ES : all)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
 }

@@ -452,3 +452,4 @@ def test_lsrk54_order3(precision):
         print(f"  state {s}: rel L2 of the change {r:.3e}", flush=True)
         assert r < tol, (precision, s, r)
     dg.close()
+
