@@ -69,7 +69,7 @@ template <int Nq> __device__ __forceinline__ int opposite_node(int n, int f) {
     }
 }
 
-// volume node of face node fn = a + Nq b of face f (the inverse of node_faces)
+// volume node of face node fn = a + Nq b of face f (the inverse of node_face_dir)
 template <int Nq> __device__ __forceinline__ int face_node(int f, int fn) {
     constexpr int N = Nq - 1;
     const int a = fn % Nq, b = fn / Nq;
@@ -121,7 +121,7 @@ __device__ __forceinline__ double own_aux(const KParams &P, const double *__rest
 // Lifted-jump term Phi(n G*) - Phi(n G-) of face node fn of face f of element e
 // (DGModel_kernels.jl:1471-1647, CentralNumericalFluxGradient NumericalFluxes.jl:67-123,
 // walls: G* = n G(q_wall), bc_momentum.jl:24-43).  G- and theta_v- come from the
-// element's own gradient arguments in LDS, q+ from the neighbour's state trace.
+// element's own gradient arguments in LDS, q+ from the neighbour's volume state.
 template <int Nq, bool SMAG>
 __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
                                                   int32_t ne, const double *__restrict__ Q,
@@ -490,8 +490,6 @@ k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *Q, const double
 
     double qn[5];
     if (act) {
-        (void)u;
-        (void)pm;
         // the face fluxes of this node's faces, in face order
 #pragma unroll
         for (int d = 0; d < 3; ++d) {

@@ -41,15 +41,7 @@ struct MlpDev {
 typedef short mlp_s16x4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
 
-__host__ __device__ inline uint16_t f32_to_bf16(float f) {
-    uint32_t u;
-    __builtin_memcpy(&u, &f, 4);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-// BF16 rounding (nearest even) by the hardware conversion; the same value as
-// f32_to_bf16 for every finite input.
+// BF16 rounding (nearest even) by the hardware conversion.
 __device__ __forceinline__ uint16_t f32_to_bf16_hw(float f) {
     const __bf16 b = (__bf16)f;
     return __builtin_bit_cast(uint16_t, b);

@@ -234,20 +234,6 @@ __device__ __forceinline__ void wall_state(const double q[5], const double nv[3]
     for (int d = 0; d < 3; ++d) qb[1 + d] = q[1 + d] - (f * dot) * nv[d];
 }
 
-// Which faces hold node (i,j,k): face f and its face-node index fn (Grids.jl:581-588).
-template <int Nq>
-__device__ __forceinline__ int node_faces(int i, int j, int k, int f[3], int fn[3]) {
-    constexpr int N = Nq - 1;
-    int c = 0;
-    if (i == 0) { f[c] = 0; fn[c] = j + Nq * k; ++c; }
-    if (i == N) { f[c] = 1; fn[c] = j + Nq * k; ++c; }
-    if (j == 0) { f[c] = 2; fn[c] = i + Nq * k; ++c; }
-    if (j == N) { f[c] = 3; fn[c] = i + Nq * k; ++c; }
-    if (k == 0) { f[c] = 4; fn[c] = i + Nq * j; ++c; }
-    if (k == N) { f[c] = 5; fn[c] = i + Nq * j; ++c; }
-    return c;
-}
-
 // Component d (0, 1, 2) of three values, chosen among values rather than among array
 // elements: `d == 0 ? a[0] : ...` on arrays is a choice of address, which keeps the
 // array in private (scratch) memory when d is only known at run time.
@@ -258,7 +244,7 @@ __device__ __forceinline__ T pick3(int d, T a, T b, T c) {
 
 // The face of direction d (0, 1, 2) that node (i, j, k) lies on -- 2d or 2d + 1 -- or
 // -1, with its face-node index fn.  Looping d = 0, 1, 2 (unrolled) visits a node's
-// faces in ascending face order, the order of node_faces, with compile-time
+// faces in ascending face order, the order of the reference's face loop, with compile-time
 // indices: per-direction arrays then stay in registers instead of private memory.
 template <int Nq>
 __device__ __forceinline__ int node_face_dir(int i, int j, int k, int d, int &fn) {
