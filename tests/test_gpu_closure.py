@@ -3,6 +3,7 @@ oracle: the MFMA network, the fused learned right-hand side, the training gradie
 and Adam step, the FP64 velocity gradient, the 2:1 filter, the subgrid samples and
 the equispaced interpolation."""
 import math
+import os
 
 import numpy as np
 import pytest
@@ -328,3 +329,31 @@ def test_equispaced_velocity():
         ix, iy, iz = grid.ex[e], grid.ey[e], grid.ez[e]
         blk = out[:, iz * p:(iz + 1) * p, iy * p:(iy + 1) * p, ix * p:(ix + 1) * p]
         assert np.allclose(blk, v, rtol=0, atol=1e-12 * np.abs(v).max())
+
+
+def test_learned_closure_beats_smagorinsky_a_posteriori(tmp_path):
+    """A-posteriori, at reduced size: a fine FP64 Taylor-Green run (32^3 elements, Mach 0.1,
+    Re 1600) filtered 2:1 onto 16^3 elements is the reference; coarse runs from the
+    filtered initial state with the chosen form (mixed precision + the committed learned
+    closure) and with the reference's Smagorinsky-Lilly (C_s = 0.21) are compared with it
+    at t* = 2 and 4 by the filtered kinetic energy and its spectrum.  The learned closure
+    must track the filtered run better than Smagorinsky-Lilly (measured: KE error 1.2e-3
+    against 8.2e-3 at t* = 4, spectrum 8.6e-4 against 4.6e-3).  On this flow the network
+    adds almost nothing to its molecular-viscosity floor: the run without a subgrid model
+    lands on the same figures, recorded here as well."""
+    import json
+    from cmdg import tgv_closure
+    print("[closure] a-posteriori 32^3 -> 16^3", flush=True)
+    weights = os.path.join(os.path.dirname(closure.__file__), "data", "tgv_closure.npz")
+    tgv_closure.main(["--stage", "eval", "--fine", "32", "--coarse", "16", "--tend", "4", "--eval-times", "2,4",
+                      "--weights", weights, "--models", "nets,smag,none", "--out", str(tmp_path)])
+    ev = json.load(open(tmp_path / "eval.json"))
+    m = ev["models"]
+    assert all(r["finite"] for r in m.values())
+    for t in ("2.0", "4.0"):
+        le, sm, no = m["learned_tgv_closure"]["times"][t], m["smag"]["times"][t], m["none"]["times"][t]
+        print(f"  t*={t}: KE err learned {le['ke_rel_err']:.3e}, smagorinsky {sm['ke_rel_err']:.3e}, "
+              f"no model {no['ke_rel_err']:.3e}; spectrum learned {le['spectrum_rel_l2']:.3e}, "
+              f"smagorinsky {sm['spectrum_rel_l2']:.3e}", flush=True)
+        assert le["ke_rel_err"] < 0.5 * sm["ke_rel_err"]
+        assert le["spectrum_rel_l2"] < 0.5 * sm["spectrum_rel_l2"]
