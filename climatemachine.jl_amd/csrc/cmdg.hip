@@ -85,7 +85,7 @@ struct cmdg_ctx {
     AffArrays AA{};
     double *d_geo = nullptr, *d_W = nullptr, *d_Wf = nullptr;
     int32_t *d_nbe = nullptr;
-    double *d_qtr[2] = {nullptr, nullptr}, *d_f2tr = nullptr, *d_str = nullptr;  // face traces
+    double *d_f2tr = nullptr, *d_str = nullptr;  // face traces
     // mixed precision (FP32 fluxes and gradient flux)
     bool mixed = false;
     MixParams MP{};
@@ -150,13 +150,6 @@ template <class T> int upload(cmdg_ctx *c, T **dst, const T *src, size_t count) 
     return CMDG_OK;
 }
 
-template <int Nq> int launch_trace_impl(cmdg_ctx *c, const double *Q, double *qtr) {
-    dim3 grid((unsigned)xcd_grid(c->nelem)), block(threads_for<Nq>());
-    TimedLaunch tl(c, KIND_COPY);
-    hipLaunchKernelGGL((k_trace_aff<Nq>), grid, block, 0, c->stream, c->nelem, Q, qtr);
-    return check_hip(c, hipGetLastError(), "trace launch");
-}
-
 // ES: the elements of this launch (affine kernels; the general ones take them all).
 template <int Nq>
 int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, const ElemSet &ES) {
@@ -188,24 +181,22 @@ int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_
 }
 
 template <int Nq>
-int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double *tend, double alpha, double beta,
-                         int mode, double *Qout, double *qtr_out, double rka, double bdt, int write_aux,
-                         const ElemSet &ES) {
+int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta, int mode,
+                         double *Qout, double rka, double bdt, int write_aux, const ElemSet &ES) {
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)xcd_grid(c->affine ? ES.count : c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_TENDENCY);
 #define CMDG_TEND_MIX(DIFF, SMAG, LEARN, MODE)                                                                   \
     hipLaunchKernelGGL((k_tendency_mix<Nq, DIFF, SMAG, LEARN, MODE>), grid, block, 0, c->stream, c->P, c->MP, c->AA, \
-                       ES, Q, qtr, c->d_gf32, c->d_f2tr32, c->d_aux, tend, alpha, beta, Qout, qtr_out, rka,   \
-                       bdt, write_aux)
+                       ES, Q, c->d_gf32, c->d_f2tr32, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux)
 #define CMDG_TEND(DIFF, SMAG, MODE)                                                                               \
     do {                                                                                                          \
         if (c->mixed)                                                                                             \
             CMDG_TEND_MIX(DIFF, SMAG, false, MODE);                                                               \
         else if (c->affine)                                                                                       \
             hipLaunchKernelGGL((k_tendency_aff<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->AA,    \
-                               ES, Q, qtr, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, qtr_out,  \
-                               rka, bdt, write_aux);                                                              \
+                               ES, Q, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, rka, bdt,      \
+                               write_aux);                                                                        \
         else                                                                                                      \
             hipLaunchKernelGGL((k_tendency<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->A, Q,      \
                                c->d_gf, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux);                 \
@@ -234,12 +225,6 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, const double *qtr, double
     default: return fail(c, CMDG_EUNSUPPORTED, "polynomial order not compiled"); \
     }
 
-int launch_trace(cmdg_ctx *c, const double *Q, double *qtr) {
-#define CALL(NQ) launch_trace_impl<NQ>(c, Q, qtr)
-    CMDG_DISPATCH_NQ(CALL)
-#undef CALL
-}
-
 int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, const ElemSet *ES = nullptr) {
     const ElemSet all{nullptr, 0, c->nelem};
 #define CALL(NQ) launch_gradient_impl<NQ>(c, Q, Qcopy, write_aux, ES ? *ES : all)
@@ -247,12 +232,10 @@ int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, 
 #undef CALL
 }
 
-int launch_tendency(cmdg_ctx *c, const double *Q, const double *qtr, double *tend, double alpha, double beta, int mode,
-                    double *Qout, double *qtr_out, double rka, double bdt, int write_aux,
-                    const ElemSet *ES = nullptr) {
+int launch_tendency(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta, int mode, double *Qout,
+                    double rka, double bdt, int write_aux, const ElemSet *ES = nullptr) {
     const ElemSet all{nullptr, 0, c->nelem};
-#define CALL(NQ) launch_tendency_impl<NQ>(c, Q, qtr, tend, alpha, beta, mode, Qout, qtr_out, rka, bdt, write_aux, \
-                                          ES ? *ES : all)
+#define CALL(NQ) launch_tendency_impl<NQ>(c, Q, tend, alpha, beta, mode, Qout, rka, bdt, write_aux, ES ? *ES : all)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
 }
@@ -261,10 +244,8 @@ int launch_tendency(cmdg_ctx *c, const double *Q, const double *qtr, double *ten
 int launch_rhs(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta) {
     int rc;
     if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
-    double *qtr = c->affine ? c->d_qtr[0] : nullptr;  // the face trace of Q (affine bricks)
-    if (c->affine && (rc = launch_trace(c, Q, qtr))) return rc;
     if (c->diffusive && (rc = launch_gradient(c, Q, nullptr, 1))) return rc;
-    return launch_tendency(c, Q, qtr, tend, alpha, beta, 0, nullptr, nullptr, 0.0, 0.0, c->diffusive ? 0 : 1);
+    return launch_tendency(c, Q, tend, alpha, beta, 0, nullptr, 0.0, 0.0, c->diffusive ? 0 : 1);
 }
 
 int launch_copy(cmdg_ctx *c, double *dst, const double *src, int64_t n) {
@@ -515,8 +496,6 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
             const char *tenv = std::getenv("CMDG_NEIGHBOUR_STATIC_TRACE");  // 1: always read it
             c->AA.aux_cont = (cont && !(tenv && tenv[0] == '1')) ? 1 : 0;
             if ((rc = upload(c, &c->d_str, st.data(), st.size()))) return rc;
-            for (int b = 0; b < 2; ++b)
-                if ((rc = upload<double>(c, &c->d_qtr[b], nullptr, (size_t)E * 6 * TR_Q * Nfp))) return rc;
             if ((rc = upload<double>(c, &c->d_f2tr, nullptr, (size_t)E * 6 * TR_F2 * Nfp))) return rc;
             c->AA.geo = c->d_geo;
             c->AA.nbe = c->d_nbe;
@@ -581,8 +560,6 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_nbe);
     (void)hipFree(c->d_W);
     (void)hipFree(c->d_Wf);
-    (void)hipFree(c->d_qtr[0]);
-    (void)hipFree(c->d_qtr[1]);
     (void)hipFree(c->d_f2tr);
     (void)hipFree(c->d_str);
     (void)hipFree(c->d_gf32);
@@ -630,52 +607,33 @@ int cmdg_lsrk54_steps(cmdg_ctx *c, double *Q, double *dQ, double t, double dt, i
     if (c->learned && !c->mlp_ready) return fail(c, CMDG_EINVAL, "learned closure weights not set");
     if (nsteps == 0) return CMDG_OK;
     int rc;
-    if (c->affine) {
-        // Affine bricks: the tendency pass reads its neighbours only through the state's
-        // face trace, never their volume state, so every stage updates Q in place; the
-        // gradient pass, which writes no state, gathers the neighbours' face values from
-        // their volume state.  The trace is ping-ponged: written from Q by the trace
-        // kernel once, then by each stage's epilogue for the next stage -- across step
-        // boundaries too, since no caller code runs between the steps of one call.
-        if ((rc = launch_trace(c, Q, c->d_qtr[0]))) return rc;
-        int cur = 0;
-        for (int64_t step = 0; step < nsteps; ++step)
-            for (int s = 0; s < 5; ++s) {
-                const double a = RKA[(s + 1) % 5], bdt = RKB[s] * dt;
-                const int wa = (s == 4);  // aux keeps the state of the last stage, as after dostep!
-                const bool last = (step == nsteps - 1) && s == 4;
-                double *qtr = c->d_qtr[cur], *qtr_out = last ? nullptr : c->d_qtr[cur ^ 1];
-                if (c->diffusive && (rc = launch_gradient(c, Q, nullptr, wa))) return rc;
-                if ((rc = launch_tendency(c, Q, qtr, dQ, 1.0, 1.0, 1, Q, qtr_out, a, bdt, c->diffusive ? 0 : wa)))
-                    return rc;
-                cur ^= 1;
-            }
-        return CMDG_OK;
-    }
-    // General geometry reads the neighbours' volume state: ping-pong between the
-    // caller's Q (C) and the context buffer (S) so that the fused stage update never
-    // overwrites a state that neighbouring elements still read: S <- C first, then
-    // the stages run S->C, C->S, S->C, C->S, S->C and the result lands in C.
+    // Every stage reads its neighbours' volume state, so it writes the next stage's
+    // state to another buffer: the stages alternate between the caller's Q (C) and the
+    // context buffer S, and no caller code runs between the steps of one call.  Over
+    // 5 nsteps stages the result lands in C when that count is even; when it is odd,
+    // S <- C first (on general meshes by the first gradient pass) and the first stage
+    // reads S.
     double *C = Q, *S = c->d_Qs;
-    const int64_t n = (int64_t)c->Np * 5 * c->nelem;
-    for (int64_t step = 0; step < nsteps; ++step) {
-        if (!c->diffusive && (rc = launch_copy(c, S, C, n))) return rc;
-        double *src = S, *dst = C;
+    double *src = C, *dst = S;
+    const bool odd = (nsteps % 2) != 0;
+    if (odd) {
+        src = S;
+        dst = C;
+        if ((c->affine || !c->diffusive) && (rc = launch_copy(c, S, C, (int64_t)c->Np * 5 * c->nelem))) return rc;
+    }
+    for (int64_t step = 0; step < nsteps; ++step)
         for (int s = 0; s < 5; ++s) {
             const double a = RKA[(s + 1) % 5], bdt = RKB[s] * dt;
-            const int wa = (s == 4);
+            const int wa = (s == 4);  // aux keeps the state of the last stage, as after dostep!
             if (c->diffusive) {
-                // the first gradient pass also leaves S <- C
-                double *Qcopy = (s == 0) ? S : nullptr;
-                if ((rc = launch_gradient(c, (s == 0) ? C : src, Qcopy, wa))) return rc;
+                const bool first = odd && step == 0 && s == 0 && !c->affine;
+                if ((rc = launch_gradient(c, first ? C : src, first ? S : nullptr, wa))) return rc;
             }
-            if ((rc = launch_tendency(c, src, nullptr, dQ, 1.0, 1.0, 1, dst, nullptr, a, bdt, c->diffusive ? 0 : wa)))
-                return rc;
+            if ((rc = launch_tendency(c, src, dQ, 1.0, 1.0, 1, dst, a, bdt, c->diffusive ? 0 : wa))) return rc;
             double *tmp = src;
             src = dst;
             dst = tmp;
         }
-    }
     return CMDG_OK;
 }
 
@@ -745,13 +703,11 @@ int cmdg_velocity_gradient(cmdg_ctx *c, const double *Q, double *grad_u) {
     if (!c) return CMDG_EINVAL;
     if (!Q || !grad_u) return fail(c, CMDG_EINVAL, "null pointer");
     if (!c->affine) return fail(c, CMDG_EUNSUPPORTED, "the velocity gradient is built for affine brick meshes");
-    int rc = launch_trace(c, Q, c->d_qtr[0]);
-    if (rc) return rc;
     dim3 grid((unsigned)xcd_grid(c->nelem));
     if (c->Nq == 5)
-        hipLaunchKernelGGL(k_velgrad_aff<5>, grid, dim3(threads_for<5>()), 0, c->stream, c->AA, c->nelem, Q, c->d_qtr[0], grad_u);
+        hipLaunchKernelGGL(k_velgrad_aff<5>, grid, dim3(threads_for<5>()), 0, c->stream, c->AA, c->nelem, Q, grad_u);
     else
-        hipLaunchKernelGGL(k_velgrad_aff<4>, grid, dim3(threads_for<4>()), 0, c->stream, c->AA, c->nelem, Q, c->d_qtr[0], grad_u);
+        hipLaunchKernelGGL(k_velgrad_aff<4>, grid, dim3(threads_for<4>()), 0, c->stream, c->AA, c->nelem, Q, grad_u);
     return check_hip(c, hipGetLastError(), "velocity gradient launch");
 }
 

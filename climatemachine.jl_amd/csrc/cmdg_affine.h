@@ -9,13 +9,13 @@
 // component of the general formula is dropped; for finite data this changes no
 // result bit.
 //
-// Neighbour data reaches a face through face traces: face-major arrays
-// (nelem, 6, nvar, Nfp) in which the 25 values a neighbour needs for one face and
-// one variable are contiguous.  The state trace is written by whoever produces
-// the state (the LSRK epilogue for the next stage, or k_trace_aff), the trace of
-// the second-order flux row by the gradient kernel, and the trace of the static
-// fields (Phi, p_ref) once at set-up.  A neighbour read is then two 128-byte lines
-// per variable instead of one line per node.
+// A face reads its neighbour's state straight from the neighbour's volume state
+// (25 values per face and variable; the kernels that update the state write it to
+// another buffer than the one they read).  Derived neighbour data reaches a face
+// through face traces: face-major arrays (nelem, 6, nvar, Nfp) in which the 25
+// values a neighbour needs for one face and one variable are contiguous -- the
+// second-order flux rows, written by the gradient kernel, and the static fields
+// (Phi, p_ref), written once at set-up.
 #pragma once
 #include "cmdg_physics.h"
 
@@ -24,7 +24,6 @@ namespace cmdg {
 // per-element record: xi11, xi22, xi33, J, sJ (6 faces), Delta
 constexpr int AFF_GEO = 11;
 // trace widths
-constexpr int TR_Q = 5;   // rho, rhou (3), rhoe
 constexpr int TR_F2 = 4;  // second-order flux row d, states 1..4 (the mass entry is 0)
 constexpr int TR_S = 2;   // Phi, p_ref
 
@@ -87,26 +86,7 @@ template <int Nq> __device__ __forceinline__ int face_node(int f, int fn) {
 // tasks t = f Nfp + fn dealt round-robin (150 tasks on 128 lanes at N = 4 take 2 + 1
 // wave-iterations, where one lane per volume node would run its up-to-3 faces in 3 +
 // 3), each task's result goes to LDS, and each node then adds the results of its
-// faces in face order (the reference's order).  Both sides of a face are read from
-// the face traces.
-
-// ------------------------------------------------------------------ state trace
-// Writes the face-node trace of Q.
-template <int Nq>
-__global__ void __launch_bounds__(threads_for<Nq>())
-k_trace_aff(int64_t nelem, const double *__restrict__ Q, double *__restrict__ qtr) {
-    constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
-    const int64_t e = xcd_element(blockIdx.x, nelem);
-    if (e >= nelem) return;
-    const int n = threadIdx.x;
-    const double *qb = Q + (int64_t)Np * 5 * e;
-    // face-major: consecutive lanes write consecutive face nodes
-    for (int t = n; t < 6 * Nfp; t += NT) {
-        const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
-#pragma unroll
-        for (int s = 0; s < 5; ++s) qtr[tri<Nfp>(e, f, s, TR_Q, fn)] = qb[Np * s + vn];
-    }
-}
+// faces in face order (the reference's order).
 
 // A static auxiliary field (Phi, p_ref) at volume node vn of element e, or dflt when
 // the model has no such field.
@@ -300,7 +280,7 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
 template <int Nq, bool DIFF, bool SMAG>
 __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
                                               int32_t ne, const double (*sQ)[Nq * Nq * Nq],
-                                              const double *__restrict__ qtr, const double *__restrict__ f2tr,
+                                              const double *__restrict__ Q, const double *__restrict__ f2tr,
                                               const double *f2own, double flux[5]) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
@@ -309,15 +289,16 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
     const bool wall = ne < 0;
     const int fo = f ^ 1;
     // the own side at the face's volume node, staged in LDS (state rows 0-4, Phi 5,
-    // p_ref 6); the neighbour's side from its trace
+    // p_ref 6); the neighbour's state from its volume
     double q[5], qp[5];
 #pragma unroll
     for (int s = 0; s < 5; ++s) q[s] = sQ[s][vn];
     const double Phi = sQ[5][vn], ref_p = sQ[6][vn];
     double Php = Phi, refpp = ref_p;
     if (!wall) {
+        const double *qb = Q + (int64_t)Np * 5 * ne + face_node<Nq>(fo, fn);
 #pragma unroll
-        for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
+        for (int s = 0; s < 5; ++s) qp[s] = qb[Np * s];
         if (!A.aux_cont) {
             Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
             refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
@@ -359,16 +340,14 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
 }
 
 // MODE 0: tend = alpha*RHS + beta*tend.   MODE 1: LSRK stage (tend is dQ):
-//   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka; the state trace of Qout is
-//   written to qtr_out when given.  Qout may be Q: a block reads only its own
-//   element's volume state (neighbours come from qtr), and all of those reads
-//   precede the workgroup barrier that comes before its writes.
+//   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka.  Qout is another buffer than Q:
+//   blocks read their neighbours' volume state.
 template <int Nq, bool DIFF, bool SMAG, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *Q, const double *__restrict__ qtr,
+k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
                const double *__restrict__ gfi, const double *__restrict__ f2tr, double *__restrict__ aux,
-               double *__restrict__ tend, double alpha, double beta, double *Qout,
-               double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
+               double *__restrict__ tend, double alpha, double beta, double *__restrict__ Qout, double rka,
+               double bdt, int write_aux) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     constexpr int NGF = SMAG ? 10 : 9;
     __shared__ double sD[Nq * Nq];
@@ -481,14 +460,13 @@ k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *Q, const double
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double fl5[5];
-        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], sQ, qtr, f2tr, sF2o[DIFF ? t : 0],
+        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, f2tr, sF2o[DIFF ? t : 0],
                                       fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];
     }
     __syncthreads();
 
-    double qn[5];
     if (act) {
         // the face fluxes of this node's faces, in face order
 #pragma unroll
@@ -508,23 +486,9 @@ k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *Q, const double
             double *qo = Qout + (int64_t)Np * 5 * e + n;
 #pragma unroll
             for (int s = 0; s < 5; ++s) {
-                qn[s] = q[s] + bdt * T[s];
-                qo[Np * s] = qn[s];
+                qo[Np * s] = q[s] + bdt * T[s];
                 tb[Np * s] = T[s] * rka;
             }
-        }
-    }
-    if (MODE == 1 && qtr_out) {
-        // the next stage's state trace, written face-major (contiguous per face), staged
-        // where the face tasks read this stage's state (all of them are done)
-        if (act)
-#pragma unroll
-            for (int s = 0; s < 5; ++s) sQ[s][n] = qn[s];
-        __syncthreads();
-        for (int t = n; t < 6 * Nfp; t += NT) {
-            const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sQ[s][vn];
         }
     }
 }

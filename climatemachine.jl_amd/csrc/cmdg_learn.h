@@ -15,9 +15,8 @@ namespace cmdg {
 // (DGModel_kernels.jl:1007-1127, 1471-1647; bc_momentum.jl:24-43).
 template <int Nq>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_velgrad_aff(AffArrays A, int64_t nelem, const double *__restrict__ Q, const double *__restrict__ qtr,
-              double *__restrict__ Gu) {
-    constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+k_velgrad_aff(AffArrays A, int64_t nelem, const double *__restrict__ Q, double *__restrict__ Gu) {
+    constexpr int Np = Nq * Nq * Nq;
     __shared__ double sD[Nq * Nq];
     __shared__ double sU[3][Np];
     const int64_t e = xcd_element(blockIdx.x, nelem);
@@ -60,9 +59,9 @@ k_velgrad_aff(AffArrays A, int64_t nelem, const double *__restrict__ Q, const do
         const int32_t ne = A.nbe[6 * e + f];
         double jump[3];
         if (ne >= 0) {
-            const int fo = f ^ 1;
-            const double rp = 1.0 / qtr[tri<Nfp>(ne, fo, 0, TR_Q, fn)];
-            for (int s = 0; s < 3; ++s) jump[s] = 0.5 * (rp * qtr[tri<Nfp>(ne, fo, 1 + s, TR_Q, fn)] - u[s]);
+            const double *qp = Q + (int64_t)Np * 5 * ne + face_node<Nq>(f ^ 1, fn);
+            const double rp = 1.0 / qp[0];
+            for (int s = 0; s < 3; ++s) jump[s] = 0.5 * (rp * qp[Np * (1 + s)] - u[s]);
         } else {
             for (int s = 0; s < 3; ++s) jump[s] = 0.0;
             jump[d] = -u[d];

@@ -311,13 +311,13 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 // Numerical flux at face node fn of face f of element e.  The own side arrives
 // precomputed by the volume phase (own[0..4]: the first-order flux row along the face's
 // axis, own[5..8]: the second-order row, own[9]: |u_d| + c); the neighbour's side is
-// evaluated here from its FP64 state trace (thermodynamics, p - p_ref in FP64), and
+// evaluated here from its FP64 volume state (thermodynamics, p - p_ref in FP64), and
 // the jump q- - q+ is formed in FP64 from the own state, staged in LDS with Phi and
 // p_ref (sQ rows 0-4, 5, 6).
 template <int Nq, bool DIFF>
 __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
                                               int f, int fn, int32_t ne, const double (*sQ)[Nq * Nq * Nq],
-                                              const double *__restrict__ qtr, const float *__restrict__ f2tr,
+                                              const double *__restrict__ Q, const float *__restrict__ f2tr,
                                               const float own[10], double p_e, float flux[5]) {
 #pragma clang fp contract(fast)
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
@@ -329,8 +329,9 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
 #pragma unroll
     for (int s = 0; s < 5; ++s) q[s] = sQ[s][vn];
     if (!wall) {
+        const double *qb = Q + (int64_t)Np * 5 * ne + face_node<Nq>(fo, fn);
 #pragma unroll
-        for (int s = 0; s < 5; ++s) qp[s] = qtr[tri<Nfp>(ne, fo, s, TR_Q, fn)];
+        for (int s = 0; s < 5; ++s) qp[s] = qb[Np * s];
         if (A.aux_cont) {
             Php = sQ[5][vn];
             refpp = sQ[6][vn];
@@ -373,18 +374,20 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     }
 }
 
+// MODE 0: tend = alpha*RHS + beta*tend.   MODE 1: LSRK stage (tend is dQ):
+//   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka.  Qout is another buffer than Q:
+//   blocks read their neighbours' volume state.
 // Six waves per SIMD at Nq = 5: the register cap this asks for (80) spills nothing there
 // (at Nq = 4 it would, so that order keeps the compiler's choice).
 template <int Nq, bool DIFF, bool SMAG, bool LEARN, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(Nq == 5 ? 6 : 1)))
-k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *Q,
-               const double *__restrict__ qtr, const float *__restrict__ gfi, const float *__restrict__ f2tr,
-               double *__restrict__ aux, double *__restrict__ tend, double alpha, double beta,
-               double *Qout, double *__restrict__ qtr_out, double rka, double bdt, int write_aux) {
+k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
+               const float *__restrict__ gfi, const float *__restrict__ f2tr, double *__restrict__ aux,
+               double *__restrict__ tend, double alpha, double beta, double *__restrict__ Qout, double rka,
+               double bdt, int write_aux) {
 
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     constexpr int NGF = LEARN ? 11 : (SMAG ? 10 : 9), NT = threads_for<Nq>();
-    static_assert(3 * 5 * sizeof(float) >= 5 * sizeof(double), "the state of the next stage is staged in sF");
     __shared__ float sD[Nq * Nq];
     __shared__ float sF[3][5][Np];
     // per face node: the own side's flux rows and wave speed from the volume phase
@@ -515,13 +518,12 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *Q
         float own[10], fl5[5];
 #pragma unroll
         for (int c = 0; c < 10; ++c) own[c] = sFc[t][c];
-        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], sQ, qtr, f2tr, own, p_e, fl5);
+        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, f2tr, own, p_e, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFc[t][s] = fl5[s];
     }
     __syncthreads();
 
-    double qn[5];
     if (act) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -542,23 +544,9 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *Q
 #pragma unroll
             for (int s = 0; s < 5; ++s) {
                 const double dd = (double)T[s] + tb[Np * s];
-                qn[s] = q[s] + bdt * dd;
-                qo[Np * s] = qn[s];
+                qo[Np * s] = q[s] + bdt * dd;
                 tb[Np * s] = dd * rka;
             }
-        }
-    }
-    if (MODE == 1 && qtr_out) {
-        // the next stage's state trace, written face-major (contiguous per face)
-        __syncthreads();
-        if (act)
-#pragma unroll
-            for (int s = 0; s < 5; ++s) sQ[s][n] = qn[s];
-        __syncthreads();
-        for (int t = n; t < 6 * Nfp; t += NT) {
-            const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qtr_out[tri<Nfp>(e, f, s, TR_Q, fn)] = sQ[s][vn];
         }
     }
 }

@@ -122,8 +122,10 @@ int cmdg_lsrk54_step(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt);
 /* nsteps consecutive LSRK54CarpenterKennedy steps from t with step dt: the loop of
  * solve!(Q, solver; numberofsteps) with no callbacks, src/Numerics/ODESolvers/
  * ODESolvers.jl:110-158 over dostep! (LowStorageRungeKuttaMethod.jl:102-144).
- * Identical to nsteps calls of cmdg_lsrk54_step; on affine bricks the state's face
- * trace passes from each step's last stage to the next step's first. */
+ * Identical to nsteps calls of cmdg_lsrk54_step.  The stages alternate the state
+ * between Q and a context buffer (each stage reads its neighbours' state while it
+ * writes the next); the result is in Q on return, and a call with an odd nsteps
+ * copies Q into the context buffer once at its start. */
 int cmdg_lsrk54_steps(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt, int64_t nsteps);
 
 /* One low-storage stage: dQ <- RHS(Q, t_stage) + dQ; Q += rkb*dt*dQ;
