@@ -40,11 +40,15 @@ struct MlpDev {
 
 typedef short mlp_s16x4 __attribute__((ext_vector_type(4)));
 typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+typedef float mlp_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 mlp_bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t mlp_u32x2 __attribute__((ext_vector_type(2)));
 
-// BF16 rounding (nearest even) by the hardware conversion.
-__device__ __forceinline__ uint16_t f32_to_bf16_hw(float f) {
-    const __bf16 b = (__bf16)f;
-    return __builtin_bit_cast(uint16_t, b);
+// BF16 rounding (nearest even) of two values by one hardware conversion: lo in the
+// low half of the result.
+__device__ __forceinline__ uint32_t bf16x2_hw(float lo, float hi) {
+    const mlp_f32x2 v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, mlp_bf16x2));
 }
 
 // SiLU v sigmoid(v) with the hardware reciprocal (1 ulp): its outputs are rounded to
@@ -58,9 +62,9 @@ __device__ __forceinline__ float mlp_silu(float v) {
 constexpr int MLP_LDS_PER_WAVE = 64 * MLP_K1;
 
 // Evaluates the network for the 64 nodes of the calling wave: lane l supplies the
-// normalised inputs x of its node and receives that node's output.  Every thread of
-// the workgroup must call this function (it holds a workgroup barrier); sX points to
-// the workgroup's LDS slab of MLP_LDS_PER_WAVE uint16 per wave.
+// normalised inputs x of its node and receives that node's output.  Every lane of the
+// wave must call this function (the wave exchanges its inputs through LDS); sX points
+// to the workgroup's LDS slab of MLP_LDS_PER_WAVE uint16 per wave.
 //
 // Tiling: the wave's 64 nodes are four column tiles of 16; hidden features are the
 // rows.  For a 16x16x16 MFMA, lane l holds A[l%16][4(l/16)..+3],
@@ -73,11 +77,14 @@ __device__ __forceinline__ float mlp_wave(const MlpDev *__restrict__ W, const fl
     uint16_t *slab = sX + (threadIdx.x >> 6) * MLP_LDS_PER_WAVE;
     uint32_t *row = reinterpret_cast<uint32_t *>(slab + lane * MLP_K1);
 #pragma unroll
-    for (int k = 0; k < MLP_IN / 2; ++k)
-        row[k] = (uint32_t)f32_to_bf16_hw(x[2 * k]) | ((uint32_t)f32_to_bf16_hw(x[2 * k + 1]) << 16);
+    for (int k = 0; k < MLP_IN / 2; ++k) row[k] = bf16x2_hw(x[2 * k], x[2 * k + 1]);
 #pragma unroll
     for (int k = MLP_IN / 2; k < MLP_K1 / 2; ++k) row[k] = 0u;
-    __syncthreads();
+    // the slab is the wave's own: LDS serves one wave's accesses in order, so the reads
+    // below see these writes once the compiler keeps them in program order
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     const mlp_s16x4 *w1 = reinterpret_cast<const mlp_s16x4 *>(W->w1);
     const mlp_s16x4 *w2 = reinterpret_cast<const mlp_s16x4 *>(W->w2);
@@ -104,8 +111,9 @@ __device__ __forceinline__ float mlp_wave(const MlpDev *__restrict__ W, const fl
 #pragma unroll
         for (int m = 0; m < 2; ++m) {
             const mlp_f32x4 h = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1[m], bx, B1[m], 0, 0, 0);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) hb[m][i] = (short)f32_to_bf16_hw(mlp_silu(h[i]));
+            const mlp_u32x2 w = {bf16x2_hw(mlp_silu(h[0]), mlp_silu(h[1])),
+                                 bf16x2_hw(mlp_silu(h[2]), mlp_silu(h[3]))};
+            hb[m] = __builtin_bit_cast(mlp_s16x4, w);
         }
         float part = 0.0f;
 #pragma unroll
