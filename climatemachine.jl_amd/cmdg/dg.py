@@ -208,6 +208,15 @@ class DGModel:
         self._check(self.lib.cmdg_geometry_kind(self._ctx, C.byref(k)), "cmdg_geometry_kind")
         return bool(k.value & 2)
 
+    @property
+    def static_aux_classes(self) -> bool:
+        """True when the affine kernels read the static auxiliary fields (Phi, grad Phi,
+        rho_ref, p_ref, Delta) from shared per-class blocks: elements whose fields agree
+        bitwise (one class per element layer on a brick with a height-only reference)."""
+        k = C.c_int32()
+        self._check(self.lib.cmdg_geometry_kind(self._ctx, C.byref(k)), "cmdg_geometry_kind")
+        return bool(k.value & 4)
+
     def set_kernel_timing(self, enable: bool):
         self._check(self.lib.cmdg_set_kernel_timing(self._ctx, int(bool(enable))), "cmdg_set_kernel_timing")
 

@@ -29,6 +29,8 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <string_view>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/cmdg.h"
@@ -86,6 +88,8 @@ struct cmdg_ctx {
     double *d_geo = nullptr, *d_W = nullptr, *d_Wf = nullptr;
     int32_t *d_nbe = nullptr;
     double *d_f2tr = nullptr, *d_str = nullptr;  // face traces
+    int32_t *d_cls = nullptr;  // static-field class of each element (null: none)
+    double *d_sa = nullptr;    // static fields by class
     // mixed precision (FP32 fluxes and gradient flux)
     bool mixed = false;
     MixParams MP{};
@@ -280,6 +284,54 @@ int fmask_vid(int Nq, int f, int fn) {
     }
 }
 
+// Classes of elements whose static auxiliary fields (Phi, grad Phi, rho_ref, p_ref,
+// Delta: what the affine kernels read through AffArrays::sa) agree bit for bit.  On a
+// brick whose reference state depends on height alone there is one class per element
+// layer.  Returns false (no classes) unless they are at least 4 times fewer than the
+// elements; rep[c] is the first element of class c.
+bool static_classes(const cmdg_desc *d, const KParams &P, int Np, std::vector<int32_t> &cls,
+                    std::vector<int64_t> &rep) {
+    const int64_t E = d->nelem;
+    int fields[7], nf = 0;
+    for (int f : {P.a_Phi, P.a_gPhi, P.a_gPhi >= 0 ? P.a_gPhi + 1 : -1, P.a_gPhi >= 0 ? P.a_gPhi + 2 : -1,
+                  P.a_ref_rho, P.a_ref_p, P.a_Delta})
+        if (f >= 0) fields[nf++] = f;
+    cls.assign(E, 0);
+    rep.clear();
+    if (nf == 0) {
+        rep.push_back(0);
+        return E >= 4;
+    }
+    auto key = [&](int64_t e, std::string &k) {
+        k.resize(sizeof(double) * Np * nf);
+        for (int i = 0; i < nf; ++i)
+            std::memcpy(&k[sizeof(double) * Np * i], d->aux_init + (int64_t)Np * ((int64_t)d->naux * e + fields[i]),
+                        sizeof(double) * Np);
+    };
+    std::unordered_map<size_t, std::vector<int32_t>> byhash;
+    std::string k, kr;
+    for (int64_t e = 0; e < E; ++e) {
+        key(e, k);
+        std::vector<int32_t> &bucket = byhash[std::hash<std::string_view>{}(k)];
+        int32_t found = -1;
+        for (int32_t c : bucket) {
+            key(rep[c], kr);
+            if (kr == k) {
+                found = c;
+                break;
+            }
+        }
+        if (found < 0) {
+            found = (int32_t)rep.size();
+            rep.push_back(e);
+            bucket.push_back(found);
+            if ((int64_t)rep.size() * 4 > E) return false;
+        }
+        cls[e] = found;
+    }
+    return true;
+}
+
 // Bitwise test that the mesh consists of axis-aligned affine bricks with conforming
 // neighbours; fills the per-element record used by the affine kernels.
 bool detect_affine(const cmdg_desc *d, int Nq, std::vector<double> &geo, std::vector<int32_t> &nbe,
@@ -459,6 +511,7 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     c->A.bnd = c->d_bnd;
     c->A.D = c->d_D;
     c->A.nelem = E;
+    std::vector<int64_t> crep;  // first element of each static-field class (empty: no classes)
     {
         std::vector<double> geo, W, Wf;
         std::vector<int32_t> nbe;
@@ -503,6 +556,30 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
             c->AA.Wf = c->d_Wf;
             c->AA.D = c->d_D;
             c->AA.str = c->d_str;
+            // static fields by class: on a brick with a reference state that depends on
+            // height alone the kernels read one layer's fields, held in cache, instead of
+            // 56 bytes a node (CMDG_STATIC_AUX_CLASSES=0 keeps one block per element)
+            c->AA.sa = c->d_aux;
+            c->AA.cls = nullptr;
+            const char *cenv = std::getenv("CMDG_STATIC_AUX_CLASSES");
+            std::vector<int32_t> cls;
+            if (!(cenv && cenv[0] == '0') && d->naux > 0 && static_classes(d, P, Np, cls, crep)) {
+                // the class's block of the aux layout; fields outside the class key are NaN
+                const int64_t nc = (int64_t)crep.size(), na = d->naux;
+                std::vector<double> sa((size_t)Np * na * nc, std::nan(""));
+                for (int64_t k = 0; k < nc; ++k)
+                    for (int f : {P.a_Phi, P.a_gPhi, P.a_gPhi >= 0 ? P.a_gPhi + 1 : -1,
+                                  P.a_gPhi >= 0 ? P.a_gPhi + 2 : -1, P.a_ref_rho, P.a_ref_p, P.a_Delta})
+                        if (f >= 0)
+                            std::memcpy(&sa[(size_t)Np * (na * k + f)], d->aux_init + (int64_t)Np * (na * crep[k] + f),
+                                        sizeof(double) * Np);
+                if ((rc = upload(c, &c->d_cls, cls.data(), cls.size()))) return rc;
+                if ((rc = upload(c, &c->d_sa, sa.data(), sa.size()))) return rc;
+                c->AA.cls = c->d_cls;
+                c->AA.sa = c->d_sa;
+            } else {
+                crep.clear();
+            }
         }
     }
     if (d->precision == CMDG_MIXED) {
@@ -527,12 +604,16 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
         if ((rc = upload<float>(c, &c->d_gf32, nullptr, (size_t)Np * c->ngf * E))) return rc;
         if (P.a_gPhi >= 0) {
             // grad Phi enters the mixed kernels in FP32 only: keep it rounded, 12 B a node
-            std::vector<float> g((size_t)Np * 3 * E);
-            for (int64_t e = 0; e < E; ++e)
+            // (per static-field class when there are classes)
+            const int64_t nb = crep.empty() ? E : (int64_t)crep.size();
+            std::vector<float> g((size_t)Np * 3 * nb);
+            for (int64_t b = 0; b < nb; ++b) {
+                const int64_t e = crep.empty() ? b : crep[b];
                 for (int a = 0; a < 3; ++a)
                     for (int n = 0; n < Np; ++n)
-                        g[((size_t)e * 3 + a) * Np + n] =
+                        g[((size_t)b * 3 + a) * Np + n] =
                             (float)d->aux_init[(int64_t)Np * ((int64_t)d->naux * e + P.a_gPhi + a) + n];
+            }
             if ((rc = upload(c, &c->d_gphi32, g.data(), g.size()))) return rc;
             c->AA.gphi32 = c->d_gphi32;
         }
@@ -562,6 +643,8 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_Wf);
     (void)hipFree(c->d_f2tr);
     (void)hipFree(c->d_str);
+    (void)hipFree(c->d_cls);
+    (void)hipFree(c->d_sa);
     (void)hipFree(c->d_gf32);
     (void)hipFree(c->d_gphi32);
     (void)hipFree(c->d_f2tr32);
@@ -860,7 +943,7 @@ int cmdg_weightedsum(cmdg_ctx *c, const double *Q, int32_t state, double *out) {
 
 int cmdg_geometry_kind(cmdg_ctx *c, int32_t *kind) {
     if (!c || !kind) return CMDG_EINVAL;
-    *kind = (c->affine ? 1 : 0) | (c->affine && c->AA.aux_cont ? 2 : 0);
+    *kind = (c->affine ? 1 : 0) | (c->affine && c->AA.aux_cont ? 2 : 0) | (c->affine && c->AA.cls ? 4 : 0);
     return CMDG_OK;
 }
 

@@ -35,8 +35,19 @@ struct AffArrays {
     const double *D;      // (Nq, Nq)
     const double *str;    // (nelem, 6, TR_S, Nfp) static trace
     int32_t aux_cont;     // 1: Phi and p_ref agree bitwise on both sides of every interior face
-    const float *gphi32;  // (Np, 3, nelem) grad Phi rounded to FP32 (mixed path; null without it)
+    // Static auxiliary fields (Phi, grad Phi, rho_ref, p_ref, Delta): elements whose
+    // static fields agree bitwise share one class, and the kernels read those fields
+    // from the class's block of sa, (Np, naux, nclass) in the aux layout.  Without
+    // classes, cls is null and sa is the aux array itself (class = element).
+    const int32_t *cls;   // (nelem) class of each element, or null
+    const double *sa;
+    const float *gphi32;  // (Np, 3, nclass) grad Phi rounded to FP32 (mixed path; null without it)
 };
+
+// the class whose static fields element e reads
+__device__ __forceinline__ int64_t static_class(const AffArrays &A, int64_t e) {
+    return A.cls ? (int64_t)A.cls[e] : e;
+}
 
 // The elements one launch works on: the range first .. first + count - 1, or, with a
 // list, list[0 .. count - 1].  Block b takes the element at position xcd_element(b, count):
@@ -105,8 +116,7 @@ __device__ __forceinline__ double own_aux(const KParams &P, const double *__rest
 template <int Nq, bool SMAG>
 __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
                                                   int32_t ne, const double *__restrict__ Q,
-                                                  const double *__restrict__ aux, const double (*sG)[Nq * Nq * Nq],
-                                                  double *dgf) {
+                                                  const double (*sG)[Nq * Nq * Nq], double *dgf) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
     const double nd = (f & 1) ? 1.0 : -1.0;
@@ -115,8 +125,9 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
     for (int s = 0; s < 5; ++s) G[s] = s < NG ? sG[s][vn] : 0.0;
     const double thv = SMAG ? G[4] : 0.0;
     double gPhi[3] = {0.0, 0.0, 0.0};
+    const int64_t ce = static_class(A, e);
     if (P.a_gPhi >= 0) {
-        const double *ab = aux + (int64_t)Np * P.naux * e + vn;
+        const double *ab = A.sa + (int64_t)Np * P.naux * ce + vn;
 #pragma unroll
         for (int c = 0; c < 3; ++c) gPhi[c] = ab[(int64_t)Np * (P.a_gPhi + c)];
     }
@@ -129,7 +140,7 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
         const int64_t qb = (int64_t)Np * 5 * ne + face_node<Nq>(fo, fn);
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = Q[qb + Np * s];
-        const double Php = A.aux_cont ? own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0) : A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
+        const double Php = A.aux_cont ? own_aux<Nq>(P, A.sa, ce, vn, P.a_Phi, -0.0) : A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
         Thermo tp = thermo(P, qp, Php);
         grad_arg<SMAG>(P, qp, tp, SMAG ? theta_v(P, tp) : 0.0, Gp);
 #pragma unroll
@@ -138,7 +149,7 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
         double q[5], qb[5];
 #pragma unroll
         for (int s = 0; s < 5; ++s) q[s] = Q[(int64_t)Np * (s + 5 * e) + vn];
-        const double Phi = own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0);
+        const double Phi = own_aux<Nq>(P, A.sa, ce, vn, P.a_Phi, -0.0);
         const double nv[3] = {d == 0 ? nd : 0.0, d == 1 ? nd : 0.0, d == 2 ? nd : 0.0};
         wall_state(q, nv, 1.0, qb);
         Thermo tb = thermo(P, qb, Phi);
@@ -182,7 +193,7 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
         const double *qb = Q + (int64_t)Np * 5 * e + n;
 #pragma unroll
         for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
-        a = load_aux(P, aux, Np, n, e);
+        a = load_aux(P, A.sa, Np, n, static_class(A, e));
         Thermo t = thermo(P, q, a.Phi);
         thv = theta_v(P, t);
         if (write_aux) {
@@ -198,7 +209,7 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
     // lifted central jumps of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double dgf[NGF];
-        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, aux, sG, dgf);
+        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, sG, dgf);
 #pragma unroll
         for (int s = 0; s < NGF; ++s) sJ[t][s] = dgf[s];
     }
@@ -278,8 +289,8 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
 // f of element e, seen from e (normal +-e_d): NumericalFluxes.jl:219-340, 670-715,
 // walls bc_momentum.jl:24-49 / bc_energy.jl:10-17 (zero second-order flux).
 template <int Nq, bool DIFF, bool SMAG>
-__device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
-                                              int32_t ne, const double (*sQ)[Nq * Nq * Nq],
+__device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays &A, int f, int fn, int32_t ne,
+                                              const double (*sQ)[Nq * Nq * Nq],
                                               const double *__restrict__ Q, const double *__restrict__ f2tr,
                                               const double *f2own, double flux[5]) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
@@ -375,7 +386,7 @@ k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
         const double *qb = Q + (int64_t)Np * 5 * e + n;
 #pragma unroll
         for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
-        a = load_aux(P, aux, Np, n, e);
+        a = load_aux(P, A.sa, Np, n, static_class(A, e));
         a.Delta = ge[10];
         Thermo t = thermo(P, q, a.Phi);
         if (write_aux) {
@@ -460,7 +471,7 @@ k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double fl5[5];
-        face_flux_aff<Nq, DIFF, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, f2tr, sF2o[DIFF ? t : 0],
+        face_flux_aff<Nq, DIFF, SMAG>(P, A, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, f2tr, sF2o[DIFF ? t : 0],
                                       fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFl[t][s] = fl5[s];

@@ -162,6 +162,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     __shared__ uint16_t sX[LEARN ? (NT / 64) * MLP_LDS_PER_WAVE : 1];
     const int64_t e = set_element(ES, blockIdx.x);
     if (e < 0) return;
+    const int64_t ce = static_class(A, e);
     __shared__ int32_t nb[6];  // the element's face neighbours
     if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
@@ -176,10 +177,10 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         const double *qb = Q + (int64_t)Np * 5 * e + n;
 #pragma unroll
         for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
-        const double *ab = aux + (int64_t)Np * P.naux * e + n;
+        const double *ab = A.sa + (int64_t)Np * P.naux * ce + n;
         if (P.a_Phi >= 0) Phi = ab[(int64_t)Np * P.a_Phi];
         if (A.gphi32) {
-            const float *gp = A.gphi32 + (int64_t)Np * 3 * e + n;
+            const float *gp = A.gphi32 + (int64_t)Np * 3 * ce + n;
             gPhi[0] = gp[0];
             gPhi[1] = gp[Np];
             gPhi[2] = gp[2 * Np];
@@ -213,7 +214,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             const int64_t qb = (int64_t)Np * 5 * ne + face_node<Nq>(fo, fn);
 #pragma unroll
             for (int s = 0; s < 5; ++s) qp[s] = Q[qb + Np * s];
-            const double Php = A.aux_cont ? own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0)
+            const double Php = A.aux_cont ? own_aux<Nq>(P, A.sa, ce, vn, P.a_Phi, -0.0)
                                           : A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
             grad_arg_f<SMAG>(P, MP, qp, thermo_d(P, qp, Php), Gp);
 #pragma unroll
@@ -225,7 +226,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 if (c == d) qb[1 + c] = 0.0;  // the gradient wall state removes the normal momentum
-            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, own_aux<Nq>(P, aux, e, vn, P.a_Phi, -0.0)), Gp);
+            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, own_aux<Nq>(P, A.sa, ce, vn, P.a_Phi, -0.0)), Gp);
 #pragma unroll
             for (int s = 0; s < 5; ++s) jump[s] = nd * (Gp[s] - Gm[s]);
         }
@@ -315,8 +316,8 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 // the jump q- - q+ is formed in FP64 from the own state, staged in LDS with Phi and
 // p_ref (sQ rows 0-4, 5, 6).
 template <int Nq, bool DIFF>
-__device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int64_t e,
-                                              int f, int fn, int32_t ne, const double (*sQ)[Nq * Nq * Nq],
+__device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int f, int fn,
+                                              int32_t ne, const double (*sQ)[Nq * Nq * Nq],
                                               const double *__restrict__ Q, const float *__restrict__ f2tr,
                                               const float own[10], double p_e, float flux[5]) {
 #pragma clang fp contract(fast)
@@ -395,6 +396,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     __shared__ float sFc[6 * Nfp][10];
     const int64_t e = set_element(ES, blockIdx.x);
     if (e < 0) return;
+    const int64_t ce = static_class(A, e);
     __shared__ int32_t nb[6];  // the element's face neighbours
     if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
@@ -413,7 +415,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         // the in-element pressure variation, not p itself.
         const double *q0 = Q + (int64_t)Np * 5 * e;
         const double qe[5] = {q0[0], q0[Np], q0[2 * Np], q0[3 * Np], q0[4 * Np]};
-        const double Phe = P.a_Phi >= 0 ? aux[(int64_t)Np * P.naux * e + (int64_t)Np * P.a_Phi] : -0.0;
+        const double Phe = P.a_Phi >= 0 ? A.sa[(int64_t)Np * P.naux * ce + (int64_t)Np * P.a_Phi] : -0.0;
         p_e = thermo_d(P, qe, Phe).p;
     }
     float qf[5], uf[3], gf[11], nu[3], gPhi[3] = {0.0f, 0.0f, 0.0f}, pf = 0.0f, pmf = 0.0f, cm = 0.0f, drho = 0.0f;
@@ -422,12 +424,12 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         const double *qb = Q + (int64_t)Np * 5 * e + n;
 #pragma unroll
         for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
-        const double *ab = aux + (int64_t)Np * P.naux * e + n;
+        const double *ab = A.sa + (int64_t)Np * P.naux * ce + n;
         if (P.a_Phi >= 0) Phi = ab[(int64_t)Np * P.a_Phi];
         if (P.a_ref_p >= 0) ref_p = ab[(int64_t)Np * P.a_ref_p];
         double ref_rho = P.a_ref_rho >= 0 ? ab[(int64_t)Np * P.a_ref_rho] : 0.0;
         if (A.gphi32) {
-            const float *gp = A.gphi32 + (int64_t)Np * 3 * e + n;
+            const float *gp = A.gphi32 + (int64_t)Np * 3 * ce + n;
             gPhi[0] = gp[0];
             gPhi[1] = gp[Np];
             gPhi[2] = gp[2 * Np];
@@ -518,7 +520,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         float own[10], fl5[5];
 #pragma unroll
         for (int c = 0; c < 10; ++c) own[c] = sFc[t][c];
-        face_flux_mix<Nq, DIFF>(P, MP, A, e, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, f2tr, own, p_e, fl5);
+        face_flux_mix<Nq, DIFF>(P, MP, A, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, f2tr, own, p_e, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFc[t][s] = fl5[s];
     }

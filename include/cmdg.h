@@ -243,9 +243,13 @@ int cmdg_weightedsum(cmdg_ctx *ctx, const double *Q, int32_t state, double *sum)
  * a bit set: bit 0 = axis-aligned affine bricks (per-element metrics, detected bitwise
  * at create; clear = general per-node metrics); bit 1 = on those bricks, Phi and p_ref
  * agree bitwise on both sides of every interior face, so the face kernels take the
- * neighbour's values from their own side instead of its static face trace.  Setting
- * CMDG_GENERAL_GEOMETRY=1 in the environment before cmdg_create clears bit 0;
- * CMDG_NEIGHBOUR_STATIC_TRACE=1 clears bit 1. */
+ * neighbour's values from their own side instead of its static face trace; bit 2 = on
+ * those bricks, the static auxiliary fields (Phi, grad Phi, rho_ref, p_ref, Delta) of
+ * many elements agree bitwise (every element layer of a brick whose reference state
+ * depends on height alone), and the kernels read them from one shared block per class
+ * instead of one per element.  Setting CMDG_GENERAL_GEOMETRY=1 in the environment
+ * before cmdg_create clears bit 0; CMDG_NEIGHBOUR_STATIC_TRACE=1 clears bit 1;
+ * CMDG_STATIC_AUX_CLASSES=0 clears bit 2. */
 int cmdg_geometry_kind(cmdg_ctx *ctx, int32_t *kind);
 
 /* Per-kernel device timing (diagnostic; no reference counterpart): when enabled,

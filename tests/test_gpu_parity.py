@@ -342,11 +342,42 @@ def test_neighbour_static_trace_paths_agree(precision):
     assert np.array_equal(out[False], out[True])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_static_aux_class_paths_agree(precision):
+    """The static auxiliary fields of the LES brick are bitwise the same in every element
+    of a layer, so the kernels read one block per layer; reading every element's own
+    fields instead (CMDG_STATIC_AUX_CLASSES=0) must give the same state bit for bit
+    after three fused LSRK54 steps."""
+    import os
+    from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
+    print(f"[parity] static aux classes {precision}", flush=True)
+    cfg, grid, aux, Q0 = _case("les_walls")
+    prec = abi.FP64 if precision == "fp64" else abi.MIXED
+    out = {}
+    for off in (False, True):
+        if off:
+            os.environ["CMDG_STATIC_AUX_CLASSES"] = "0"
+        try:
+            dg = DGModel(cfg, grid, aux=aux, precision=prec)
+        finally:
+            os.environ.pop("CMDG_STATIC_AUX_CLASSES", None)
+        assert dg.geometry_kind == "affine"
+        assert dg.static_aux_classes == (not off)
+        Q = to_device(Q0)
+        solver = LSRK54CarpenterKennedy(dg, Q, dt=5e-3, t0=0.0)
+        solve(Q, solver, numberofsteps=3, timeend=math.inf)
+        out[off] = (to_host(Q), to_host(dg.state_auxiliary))
+        dg.close()
+    assert np.isfinite(out[False][0]).all()
+    assert np.array_equal(out[False][0], out[True][0])
+    assert np.array_equal(out[False][1], out[True][1])  # theta_v and air_T written per element
+
+
 @pytest.mark.parametrize("variant", ["fp64_affine", "fp64_general", "mixed", "euler_affine"])
 def test_multistep_matches_single_steps(variant):
-    """cmdg_lsrk54_steps (the solve! loop without callbacks: in-place stages, the face
-    trace carried across steps) gives bit for bit the state of the same number of
-    single dostep! calls."""
+    """cmdg_lsrk54_steps (the solve! loop without callbacks; the stages alternate the
+    state between two buffers across the steps of the call) gives bit for bit the state
+    of the same number of single dostep! calls."""
     import os
     from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
     print(f"[parity] multistep {variant}", flush=True)
