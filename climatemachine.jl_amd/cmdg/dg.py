@@ -217,6 +217,14 @@ class DGModel:
         self._check(self.lib.cmdg_geometry_kind(self._ctx, C.byref(k)), "cmdg_geometry_kind")
         return bool(k.value & 4)
 
+    @property
+    def tiled_traversal(self) -> bool:
+        """True when the affine launches walk the elements in lattice tiles (so that a
+        block's face neighbours ran shortly before it) rather than in storage order."""
+        k = C.c_int32()
+        self._check(self.lib.cmdg_geometry_kind(self._ctx, C.byref(k)), "cmdg_geometry_kind")
+        return bool(k.value & 8)
+
     def set_kernel_timing(self, enable: bool):
         self._check(self.lib.cmdg_set_kernel_timing(self._ctx, int(bool(enable))), "cmdg_set_kernel_timing")
 

@@ -22,6 +22,7 @@
 // (:1769-1825) and the LSRK update! kernel (LowStorageRungeKuttaMethod.jl:146-158).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -90,6 +91,7 @@ struct cmdg_ctx {
     double *d_f2tr = nullptr, *d_str = nullptr;  // face traces
     int32_t *d_cls = nullptr;  // static-field class of each element (null: none)
     double *d_sa = nullptr;    // static fields by class
+    int32_t *d_order = nullptr;  // traversal order of the affine launches (null: storage order)
     // mixed precision (FP32 fluxes and gradient flux)
     bool mixed = false;
     MixParams MP{};
@@ -230,7 +232,7 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alph
     }
 
 int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, const ElemSet *ES = nullptr) {
-    const ElemSet all{nullptr, 0, c->nelem};
+    const ElemSet all{c->d_order, 0, c->nelem};
 #define CALL(NQ) launch_gradient_impl<NQ>(c, Q, Qcopy, write_aux, ES ? *ES : all)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
@@ -238,7 +240,7 @@ int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, 
 
 int launch_tendency(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta, int mode, double *Qout,
                     double rka, double bdt, int write_aux, const ElemSet *ES = nullptr) {
-    const ElemSet all{nullptr, 0, c->nelem};
+    const ElemSet all{c->d_order, 0, c->nelem};
 #define CALL(NQ) launch_tendency_impl<NQ>(c, Q, tend, alpha, beta, mode, Qout, rka, bdt, write_aux, ES ? *ES : all)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
@@ -329,6 +331,61 @@ bool static_classes(const cmdg_desc *d, const KParams &P, int Np, std::vector<in
         }
         cls[e] = found;
     }
+    return true;
+}
+
+// Traversal order of the affine kernels' launches.  A block gathers its face
+// neighbours' state from their volume; when the neighbour's block ran shortly before
+// on the same XCD those lines are still in that XCD's L2.  The storage order of a
+// stacked brick puts the y neighbours a whole x-y row of columns apart (625 elements
+// at 25^3, 9216 at 96^3: tens of MB, far beyond the 4 MB L2), so launches walk the
+// elements in T x T x T tiles instead: z fastest inside a tile, then x, then y, and
+// the tiles in the same order -- every neighbour inside a tile is at most T^2
+// positions away.  The lattice position of each element comes from its centroid
+// (vgeo x1, x2, x3); returns false, keeping the storage order, when the centroids do
+// not form a complete lattice or the descriptor has no coordinates.
+bool tiled_order(const cmdg_desc *d, int Np, int T, std::vector<int32_t> &order) {
+    const int64_t E = d->nelem;
+    if (T <= 0 || d->nvgeo < 15 || E > INT32_MAX) return false;
+    std::vector<int64_t> idx[3];
+    int64_t nax[3];
+    for (int a = 0; a < 3; ++a) {
+        std::vector<double> c(E);
+        for (int64_t e = 0; e < E; ++e) {
+            const double *x = d->vgeo + (int64_t)Np * ((int64_t)d->nvgeo * e + 12 + a);
+            double sum = 0.0;
+            for (int n = 0; n < Np; ++n) sum += x[n];
+            c[e] = sum / Np;
+        }
+        std::vector<double> u(c);
+        std::sort(u.begin(), u.end());
+        const double tol = 1e-9 * (std::fabs(u.front()) + std::fabs(u.back()) + 1e-300);
+        std::vector<double> lev;  // one value per lattice plane
+        for (double v : u)
+            if (lev.empty() || v - lev.back() > tol) lev.push_back(v);
+        nax[a] = (int64_t)lev.size();
+        idx[a].resize(E);
+        for (int64_t e = 0; e < E; ++e) {
+            auto it = std::lower_bound(lev.begin(), lev.end(), c[e] - tol);
+            if (it == lev.end() || std::fabs(*it - c[e]) > tol) return false;
+            idx[a][e] = it - lev.begin();
+        }
+    }
+    if (nax[0] * nax[1] * nax[2] != E) return false;
+    const int64_t ntx = (nax[0] + T - 1) / T, ntz = (nax[2] + T - 1) / T;
+    std::vector<std::pair<int64_t, int32_t>> key(E);
+    std::vector<char> seen(E, 0);
+    for (int64_t e = 0; e < E; ++e) {
+        const int64_t ix = idx[0][e], iy = idx[1][e], iz = idx[2][e];
+        const int64_t cell = iz + nax[2] * (ix + nax[0] * iy);
+        if (seen[cell]) return false;  // two elements at one lattice point
+        seen[cell] = 1;
+        const int64_t tile = iz / T + ntz * (ix / T + ntx * (iy / T));
+        key[e] = {((tile * T + iy % T) * T + ix % T) * T + iz % T, (int32_t)e};
+    }
+    std::sort(key.begin(), key.end());
+    order.resize(E);
+    for (int64_t p = 0; p < E; ++p) order[p] = key[p].second;
     return true;
 }
 
@@ -580,6 +637,16 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
             } else {
                 crep.clear();
             }
+            // tiled traversal (CMDG_ELEMENT_TILE=T sets the tile edge, 0 the storage
+            // order).  Measured on one box: 8^3 tiles make the mixed-precision stage
+            // 0.3 % (25^3) to 2.4 % (96^3) faster and the FP64 one 1.5 % slower, so they
+            // are the mixed path's default only.
+            const char *tile_env = std::getenv("CMDG_ELEMENT_TILE");
+            const int T = tile_env ? std::atoi(tile_env) : (d->precision == CMDG_MIXED ? 8 : 0);
+            std::vector<int32_t> order;
+            if (tiled_order(d, Np, T, order)) {
+                if ((rc = upload(c, &c->d_order, order.data(), order.size()))) return rc;
+            }
         }
     }
     if (d->precision == CMDG_MIXED) {
@@ -645,6 +712,7 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_str);
     (void)hipFree(c->d_cls);
     (void)hipFree(c->d_sa);
+    (void)hipFree(c->d_order);
     (void)hipFree(c->d_gf32);
     (void)hipFree(c->d_gphi32);
     (void)hipFree(c->d_f2tr32);
@@ -943,7 +1011,8 @@ int cmdg_weightedsum(cmdg_ctx *c, const double *Q, int32_t state, double *out) {
 
 int cmdg_geometry_kind(cmdg_ctx *c, int32_t *kind) {
     if (!c || !kind) return CMDG_EINVAL;
-    *kind = (c->affine ? 1 : 0) | (c->affine && c->AA.aux_cont ? 2 : 0) | (c->affine && c->AA.cls ? 4 : 0);
+    *kind = (c->affine ? 1 : 0) | (c->affine && c->AA.aux_cont ? 2 : 0) | (c->affine && c->AA.cls ? 4 : 0) |
+            (c->affine && c->d_order ? 8 : 0);
     return CMDG_OK;
 }
 
