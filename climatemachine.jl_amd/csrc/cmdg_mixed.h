@@ -310,8 +310,10 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 
 // ------------------------------------------------------------------ tendency (mixed)
 // Numerical flux at face node fn of face f of element e.  The own side arrives
-// precomputed by the volume phase (own[0..4]: the first-order flux row along the face's
-// axis, own[5..8]: the second-order row, own[9]: |u_d| + c); the neighbour's side is
+// precomputed by the volume phase (own[0..3]: entries 1-4 of the first-order flux row
+// along the face's axis, own[4..7]: the second-order row, own[8]: |u_d| + c; the row's
+// mass entry is the FP32 rounding of rho u_d, taken here from the staged state); the
+// neighbour's side is
 // evaluated here from its FP64 volume state (thermodynamics, p - p_ref in FP64), and
 // the jump q- - q+ is formed in FP64 from the own state, staged in LDS with Phi and
 // p_ref (sQ rows 0-4, 5, 6).
@@ -319,7 +321,7 @@ template <int Nq, bool DIFF>
 __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int f, int fn,
                                               int32_t ne, const double (*sQ)[Nq * Nq * Nq],
                                               const double *__restrict__ Q, const float *__restrict__ f2tr,
-                                              const float own[10], double p_e, float flux[5]) {
+                                              const float own[9], double p_e, float flux[5]) {
 #pragma clang fp contract(fast)
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     const int d = f >> 1, vn = face_node<Nq>(f, fn);
@@ -361,17 +363,18 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     for (int c = 0; c < 3; ++c) upf[c] = qpf[1 + c] * rip;
     float Fp[5];
     flux1_row_f(qpf, upf, (float)tp.p, (float)(MP.ref_sub ? tp.p - refpp : tp.p - p_e), d, Fp);
+    flux[0] = (pick3(d, (float)q[1], (float)q[2], (float)q[3]) + Fp[0]) * h;
 #pragma unroll
-    for (int s = 0; s < 5; ++s) flux[s] = (own[s] + Fp[s]) * h;
+    for (int s = 1; s < 5; ++s) flux[s] = (own[s - 1] + Fp[s]) * h;
     if (MP.rusanov) {
         const float upd = pick3(d, upf[0], upf[1], upf[2]);
-        const float mw = 0.5f * fmaxf(own[9], fabsf(upd) + sqrtf(MP.gammaR * (float)tp.temp));
+        const float mw = 0.5f * fmaxf(own[8], fabsf(upd) + sqrtf(MP.gammaR * (float)tp.temp));
 #pragma unroll
         for (int s = 0; s < 5; ++s) flux[s] += mw * jump[s];
     }
     if (DIFF && !wall) {
 #pragma unroll
-        for (int v = 0; v < TR_F2; ++v) flux[1 + v] += (own[5 + v] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
+        for (int v = 0; v < TR_F2; ++v) flux[1 + v] += (own[4 + v] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
     }
 }
 
@@ -392,8 +395,9 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     __shared__ float sD[Nq * Nq];
     __shared__ float sF[3][5][Np];
     // per face node: the own side's flux rows and wave speed from the volume phase
-    // (first-order row 5, second-order row 4, |u_d| + c), then the face flux (first 5)
-    __shared__ float sFc[6 * Nfp][10];
+    // (first-order row entries 1-4, second-order row 4, |u_d| + c), then the face flux
+    // (first 5); 9 floats keep the block at 13 KB of LDS, 12 blocks (6 waves/SIMD) a CU
+    __shared__ float sFc[6 * Nfp][9];
     const int64_t e = set_element(ES, blockIdx.x);
     if (e < 0) return;
     const int64_t ce = static_class(A, e);
@@ -467,14 +471,14 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             const int f = node_face_dir<Nq>(i, j, k, ax, fn);
             if (f >= 0) {
 #pragma unroll
-                for (int s = 0; s < 5; ++s) sFc[f * Nfp + fn][s] = F[s];
-                sFc[f * Nfp + fn][9] = fabsf(uf[ax]) + cm;
+                for (int s = 1; s < 5; ++s) sFc[f * Nfp + fn][s - 1] = F[s];
+                sFc[f * Nfp + fn][8] = fabsf(uf[ax]) + cm;
             }
             if (DIFF) {
                 flux2_row_f<SMAG>(MP, qf, gf, nu, ax, F2);
                 if (f >= 0)
 #pragma unroll
-                    for (int v = 0; v < TR_F2; ++v) sFc[f * Nfp + fn][5 + v] = F2[1 + v];
+                    for (int v = 0; v < TR_F2; ++v) sFc[f * Nfp + fn][4 + v] = F2[1 + v];
 #pragma unroll
                 for (int s = 0; s < 5; ++s) F[s] += F2[s];
             }
@@ -517,9 +521,9 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     __syncthreads();
     // numerical fluxes of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
-        float own[10], fl5[5];
+        float own[9], fl5[5];
 #pragma unroll
-        for (int c = 0; c < 10; ++c) own[c] = sFc[t][c];
+        for (int c = 0; c < 9; ++c) own[c] = sFc[t][c];
         face_flux_mix<Nq, DIFF>(P, MP, A, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, f2tr, own, p_e, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFc[t][s] = fl5[s];
