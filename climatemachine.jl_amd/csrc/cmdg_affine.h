@@ -66,31 +66,16 @@ template <int Nfp> __device__ __forceinline__ int64_t tri(int64_t e, int f, int 
     return ((e * 6 + f) * nv + v) * (int64_t)Nfp + fn;
 }
 
-// volume node of the neighbour that faces node n across face f (conforming bricks)
-template <int Nq> __device__ __forceinline__ int opposite_node(int n, int f) {
-    constexpr int N = Nq - 1;
-    switch (f) {
-    case 0: return n + N;
-    case 1: return n - N;
-    case 2: return n + N * Nq;
-    case 3: return n - N * Nq;
-    case 4: return n + N * Nq * Nq;
-    default: return n - N * Nq * Nq;
-    }
-}
-
-// volume node of face node fn = a + Nq b of face f (the inverse of node_face_dir)
+// volume node of face node fn = a + Nq b of face f (the inverse of node_face_dir): the
+// coordinate along the face's normal is 0 or N, the other two are (a, b) in order.
+// Selects rather than a switch: the face tasks of one wave lie on up to three faces, and
+// a switch over f branches once per face present.
 template <int Nq> __device__ __forceinline__ int face_node(int f, int fn) {
-    constexpr int N = Nq - 1;
-    const int a = fn % Nq, b = fn / Nq;
-    switch (f) {
-    case 0: return Nq * (a + Nq * b);
-    case 1: return N + Nq * (a + Nq * b);
-    case 2: return a + Nq * Nq * b;
-    case 3: return a + Nq * (N + Nq * b);
-    case 4: return a + Nq * b;
-    default: return a + Nq * (b + Nq * N);
-    }
+    const int a = fn % Nq, b = fn / Nq, d = f >> 1, o = (f & 1) * (Nq - 1);
+    const int i = d == 0 ? o : a;
+    const int j = d == 1 ? o : (d == 0 ? a : b);
+    const int k = d == 2 ? o : b;
+    return i + Nq * (j + Nq * k);
 }
 
 // Face work is spread over all lanes of the workgroup: the 6 Nfp face nodes are
