@@ -219,8 +219,9 @@ class DGModel:
 
     @property
     def tiled_traversal(self) -> bool:
-        """True when the affine launches walk the elements in lattice tiles (so that a
-        block's face neighbours ran shortly before it) rather than in storage order."""
+        """True when an affine pass (by default the tendency pass) walks the elements in
+        lattice tiles (so that a block's face neighbours ran shortly before it) rather
+        than in storage order."""
         k = C.c_int32()
         self._check(self.lib.cmdg_geometry_kind(self._ctx, C.byref(k)), "cmdg_geometry_kind")
         return bool(k.value & 8)

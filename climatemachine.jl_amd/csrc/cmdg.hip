@@ -91,7 +91,8 @@ struct cmdg_ctx {
     double *d_f2tr = nullptr, *d_str = nullptr;  // face traces
     int32_t *d_cls = nullptr;  // static-field class of each element (null: none)
     double *d_sa = nullptr;    // static fields by class
-    int32_t *d_order = nullptr;  // traversal order of the affine launches (null: storage order)
+    int32_t *d_order = nullptr;  // tiled traversal order of the affine launches (null: none)
+    const int32_t *ord_grad = nullptr, *ord_tend = nullptr;  // per pass: d_order or null (storage)
     // mixed precision (FP32 fluxes and gradient flux)
     bool mixed = false;
     MixParams MP{};
@@ -232,7 +233,7 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alph
     }
 
 int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, const ElemSet *ES = nullptr) {
-    const ElemSet all{c->d_order, 0, c->nelem};
+    const ElemSet all{c->ord_grad, 0, c->nelem};
 #define CALL(NQ) launch_gradient_impl<NQ>(c, Q, Qcopy, write_aux, ES ? *ES : all)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
@@ -240,7 +241,7 @@ int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, 
 
 int launch_tendency(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta, int mode, double *Qout,
                     double rka, double bdt, int write_aux, const ElemSet *ES = nullptr) {
-    const ElemSet all{c->d_order, 0, c->nelem};
+    const ElemSet all{c->ord_tend, 0, c->nelem};
 #define CALL(NQ) launch_tendency_impl<NQ>(c, Q, tend, alpha, beta, mode, Qout, rka, bdt, write_aux, ES ? *ES : all)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
@@ -637,15 +638,22 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
             } else {
                 crep.clear();
             }
-            // tiled traversal (CMDG_ELEMENT_TILE=T sets the tile edge, 0 the storage
-            // order).  Measured on one box: 8^3 tiles make the mixed-precision stage
-            // 0.3 % (25^3) to 2.4 % (96^3) faster and the FP64 one 1.5 % slower, so they
-            // are the mixed path's default only.
+            // tiled traversal (CMDG_ELEMENT_TILE=T sets the tile edge, 0 keeps the storage
+            // order; CMDG_TILED_PASSES = tendency, gradient, both or none chooses the
+            // passes that walk the tiles).  Measured on one box: tiles for the tendency
+            // pass alone are the fastest of the four for both precisions -- mixed + learned
+            // closure +1.8 % at 25^3 against storage order and +2.3 % at 96^3 against tiles
+            // for both passes, FP64 +1.8 % at 96^3 against storage order (within noise at
+            // 25^3).  The gradient pass loses in tiles about what its gathers gain.
             const char *tile_env = std::getenv("CMDG_ELEMENT_TILE");
-            const int T = tile_env ? std::atoi(tile_env) : (d->precision == CMDG_MIXED ? 8 : 0);
+            const int T = tile_env ? std::atoi(tile_env) : 8;
             std::vector<int32_t> order;
             if (tiled_order(d, Np, T, order)) {
                 if ((rc = upload(c, &c->d_order, order.data(), order.size()))) return rc;
+                const char *penv = std::getenv("CMDG_TILED_PASSES");
+                const std::string passes = penv ? penv : "tendency";
+                if (passes == "both" || passes == "gradient") c->ord_grad = c->d_order;
+                if (passes == "both" || passes == "tendency") c->ord_tend = c->d_order;
             }
         }
     }
@@ -1012,7 +1020,7 @@ int cmdg_weightedsum(cmdg_ctx *c, const double *Q, int32_t state, double *out) {
 int cmdg_geometry_kind(cmdg_ctx *c, int32_t *kind) {
     if (!c || !kind) return CMDG_EINVAL;
     *kind = (c->affine ? 1 : 0) | (c->affine && c->AA.aux_cont ? 2 : 0) | (c->affine && c->AA.cls ? 4 : 0) |
-            (c->affine && c->d_order ? 8 : 0);
+            (c->affine && (c->ord_grad || c->ord_tend) ? 8 : 0);
     return CMDG_OK;
 }
 

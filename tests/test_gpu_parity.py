@@ -416,10 +416,12 @@ def test_traversal_orders_agree(precision):
     out = {}
     for tile in ("0", "2"):
         os.environ["CMDG_ELEMENT_TILE"] = tile
+        os.environ["CMDG_TILED_PASSES"] = "both"
         try:
             dg = DGModel(cfg, grid, aux=aux, precision=prec)
         finally:
             os.environ.pop("CMDG_ELEMENT_TILE", None)
+            os.environ.pop("CMDG_TILED_PASSES", None)
         assert dg.tiled_traversal == (tile != "0")
         Q = to_device(Q0)
         solver = LSRK54CarpenterKennedy(dg, Q, dt=5e-3, t0=0.0)
