@@ -247,8 +247,10 @@ def hbm_traffic(kernel_prefix, workload, ne, timeout_s=150):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    # at least 100 timed steps after 10 warm-up steps, the measurement the GDOF/s figure is
+    # defined on
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", choices=("les25", "tgv96"), default="les25")
     ap.add_argument("--elements", type=int, default=0, help="elements per direction (default 25 / 96)")
     ap.add_argument("--cpu-steps", type=int, default=40)
