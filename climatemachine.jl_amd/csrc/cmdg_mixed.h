@@ -143,8 +143,9 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 }
 
 // ------------------------------------------------------------------ gradient (mixed)
-// Five waves per SIMD: the kernel waits on memory more than it computes, and the
-// register cap this asks for (96) spills nothing at Nq = 5.
+// Five waves per SIMD: the Smagorinsky-Lilly variant waits on memory more than it
+// computes (the learned one is VALU-bound), and the register cap this asks for (96)
+// spills nothing at Nq = 5.
 template <int Nq, bool SMAG, bool LEARN>
 __global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(5)))
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
