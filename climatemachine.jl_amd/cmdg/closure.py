@@ -1,4 +1,4 @@
-"""Learned eddy-viscosity closure and the tools that train and judge it.
+"""Learned eddy-viscosity closure and the tools that train and evaluate it.
 
 The closure replaces nu_0 of SmagorinskyLilly
 (src/Common/TurbulenceClosures/TurbulenceClosures.jl:478-491) by

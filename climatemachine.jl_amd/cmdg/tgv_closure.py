@@ -1,5 +1,5 @@
 """Taylor-Green closure experiment: train the learned closure on a filtered fine run
-and judge it by filtered kinetic-energy spectra.
+and evaluate it by filtered kinetic-energy spectra.
 
 Stage ``train``: a plain FP64 run on the fine brick (constant kinematic viscosity
 nu = U0 L / Re, seeded phase shift of the initial vortex) is filtered 2:1 onto the
