@@ -2,7 +2,9 @@
 grid closed forms, LSRK54 order, discrete hydrostatic balance (reference known
 answer), free-stream preservation, conservation, and the Smagorinsky-Lilly closure
 on linear velocity fields (closed form)."""
+import json
 import math
+import os
 
 import numpy as np
 import pytest
@@ -10,6 +12,8 @@ import pytest
 from cmdg import abi, atmos
 from cmdg.grid import BrickGrid, lglpoints, spectralderivative, uniform_brick
 from oracle import oracle as O
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
 
 
 # ------------------------------------------------------------------ grid
@@ -73,7 +77,8 @@ def test_lsrk54_coefficients_give_order_4():
         return abs(q - math.exp(math.sin(20.0)))
 
     e1, e2 = run(2.0 ** -6), run(2.0 ** -7)
-    assert abs(math.log2(e1 / e2) - 4) < 0.17
+    kat = KATS["lsrk54_convergence"]
+    assert abs(math.log2(e1 / e2) - kat["expected_order"]) < kat["tolerance"]
 
 
 # ------------------------------------------------------------------ balance / invariants
@@ -101,7 +106,7 @@ def test_discrete_hydrostatic_balance(profile):
     O.solve(orc, Q, dt, 100.0)
     M = grid.mass
     err = O.euclidean_distance(Q, Q0, M) / math.sqrt((Q0 ** 2 * M[:, None, :]).sum())
-    assert err <= 100 * np.finfo(np.float64).eps, err
+    assert err <= KATS["discrete_hydrostatic_balance"]["bound_in_eps"] * np.finfo(np.float64).eps, err
 
 
 def test_product_aux_init_matches_oracle():
