@@ -159,7 +159,6 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     __shared__ float sD[Nq * Nq];
     __shared__ float sG[NG][Np];
     __shared__ float sJ[6 * Nfp][5];
-    __shared__ float sF2[3][TR_F2][Np];
     __shared__ uint16_t sX[LEARN ? (NT / 64) * MLP_LDS_PER_WAVE : 1];
     const int64_t e = set_element(ES, blockIdx.x);
     if (e < 0) return;
@@ -283,7 +282,8 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         float *go = gfo + (int64_t)Np * NGF * e + n;
 #pragma unroll
         for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
-        // second-order flux rows of the face nodes, staged by direction
+        // second-order flux rows of the face nodes, written straight to the face-major
+        // trace: the 25 lanes of one face fill its 25 contiguous slots per row entry
         if (i == 0 || i == Nq - 1 || j == 0 || j == Nq - 1 || k == 0 || k == Nq - 1) {
             float nu[3], qf[5];
 #pragma unroll
@@ -292,20 +292,14 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 int fn;
-                if (node_face_dir<Nq>(i, j, k, d, fn) < 0) continue;
+                const int f = node_face_dir<Nq>(i, j, k, d, fn);
+                if (f < 0) continue;
                 float F2[5];
                 flux2_row_f<SMAG>(MP, qf, gf, nu, d, F2);
 #pragma unroll
-                for (int v = 0; v < TR_F2; ++v) sF2[d][v][n] = F2[1 + v];
+                for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = F2[1 + v];
             }
         }
-    }
-    __syncthreads();
-    // ... written to the trace face-major
-    for (int t = n; t < 6 * Nfp; t += NT) {
-        const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
-#pragma unroll
-        for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = sF2[f >> 1][v][vn];
     }
 }
 
