@@ -159,9 +159,8 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
     constexpr int NG = SMAG ? 5 : 4, NGF = SMAG ? 10 : 9;
     __shared__ double sD[Nq * Nq];
     __shared__ double sG[5][Np];
-    // lifted-jump terms (6 Nfp x NGF), later the F2 rows by direction (3 x TR_F2 x Np)
-    constexpr int NJ = 6 * Nfp * NGF > 3 * TR_F2 * Np ? 6 * Nfp * NGF : 3 * TR_F2 * Np;
-    __shared__ double sJbuf[NJ];
+    // lifted-jump terms (6 Nfp x NGF)
+    __shared__ double sJbuf[6 * Nfp * NGF];
     double(*sJ)[NGF] = reinterpret_cast<double(*)[NGF]>(sJbuf);
     const int64_t e = set_element(ES, blockIdx.x);
     if (e < 0) return;
@@ -244,28 +243,21 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
 #pragma unroll
         for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
     }
-    // the second-order flux rows of the face nodes, staged by direction, then
-    // written to the trace face-major
-    __syncthreads();
-    double(*sF2)[TR_F2][Np] = reinterpret_cast<double(*)[TR_F2][Np]>(sJbuf);
+    // the second-order flux rows of the face nodes, written straight to the face-major
+    // trace: the 25 lanes of one face fill its 25 contiguous slots per row entry
     if (act && (i == 0 || i == Nq - 1 || j == 0 || j == Nq - 1 || k == 0 || k == Nq - 1)) {
         double nu[3];
         viscosity<SMAG>(P, q[0], gf, a.gPhi, ge[10], nu);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             int fn;
-            if (node_face_dir<Nq>(i, j, k, d, fn) < 0) continue;
+            const int f = node_face_dir<Nq>(i, j, k, d, fn);
+            if (f < 0) continue;
             double F2[5];
             flux2_row<SMAG>(P, q, gf, nu, d, F2);
 #pragma unroll
-            for (int v = 0; v < TR_F2; ++v) sF2[d][v][n] = F2[1 + v];
+            for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = F2[1 + v];
         }
-    }
-    __syncthreads();
-    for (int t = n; t < 6 * Nfp; t += NT) {
-        const int f = t / Nfp, fn = t % Nfp, vn = face_node<Nq>(f, fn);
-#pragma unroll
-        for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = sF2[f >> 1][v][vn];
     }
 }
 
