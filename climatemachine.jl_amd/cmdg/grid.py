@@ -15,7 +15,10 @@ arrays.  Elements are stacked: the vertical index runs fastest, as in
 ``StackedBrickTopology`` (src/Numerics/Mesh/Topologies.jl:632-).  Metric terms
 of an axis-aligned brick element are the exact affine values (d xi / d x =
 2/h), the limit of the curl-invariant construction of
-src/Numerics/Mesh/Metrics.jl:431-722 for straight-sided hexahedra.
+src/Numerics/Mesh/Metrics.jl:431-722 for straight-sided hexahedra.  With a
+``meshwarp`` (the reference grid's keyword, Grids.jl:267-325, 1028-1124) the
+node coordinates are warped and every metric term comes from the curl-invariant
+construction itself (``curvilinear_geometry``).
 """
 from __future__ import annotations
 
@@ -109,17 +112,117 @@ def face_vids(Nq: int) -> np.ndarray:
     ]).astype(np.int64)
 
 
+def _d_along(D: np.ndarray, f: np.ndarray, axis: int) -> np.ndarray:
+    """sum_n D[i, n] f[.., n, ..] along reference axis `axis` of f (Nq, Nq, Nq, E),
+    accumulated over n in order as the reference's loops do."""
+    Nq = D.shape[0]
+    out = np.zeros_like(f)
+    for n in range(Nq):
+        sl = [slice(None)] * 4
+        sl[axis] = slice(n, n + 1)
+        shape = [1, 1, 1, 1]
+        shape[axis] = Nq
+        out += D[:, n].reshape(shape) * f[tuple(sl)]
+    return out
+
+
+def curvilinear_geometry(x: np.ndarray, D: np.ndarray, w: np.ndarray, fmask: np.ndarray):
+    """vgeo (Np, 25, E) and sgeo (5, Nfp, 6, E) of hexahedra with node coordinates
+    x (3, Np, E) (node n = i + Nq j + Nq^2 k), as computegeometry builds them
+    (Grids.jl:1028-1124): x_xi by the derivative matrix
+    (compute_reference_to_physical_coord_jacobian!, Metrics.jl:215-266), the
+    curl-invariant d xi / d x = D (x_m d x_l / d xi) / 2J and then x_xi replaced by
+    the inverse of that discrete metric (computemetric!, Metrics.jl:431-722), face
+    normals and surface Jacobians from J d xi_a / d x at the face nodes, mass
+    matrices with the LGL weights, and the horizontal mass (horizontal_metrics!,
+    Grids.jl:1135-1157)."""
+    Nq = D.shape[0]
+    Np, Nfp = Nq ** 3, Nq * Nq
+    E = x.shape[2]
+    X = [x[d].reshape(Nq, Nq, Nq, E, order="F") for d in range(3)]
+    # xr[d][a] = d x_d / d xi_a
+    xr = [[_d_along(D, X[d], a) for a in range(3)] for d in range(3)]
+    JcV = np.sqrt(xr[0][2] ** 2 + xr[1][2] ** 2 + xr[2][2] ** 2)
+    J = (xr[0][0] * (xr[1][1] * xr[2][2] - xr[2][1] * xr[1][2])
+         + xr[1][0] * (xr[2][1] * xr[0][2] - xr[0][1] * xr[2][2])
+         + xr[2][0] * (xr[0][1] * xr[1][2] - xr[1][1] * xr[0][2]))
+    JI2 = 1.0 / (2 * J)
+    # cross terms c[l][a] = x_m d x_n / d xi_a - x_n d x_m / d xi_a for the cyclic
+    # (l, m, n) = (1, 2, 3), (2, 3, 1), (3, 1, 2)
+    cyc = ((1, 2), (2, 0), (0, 1))
+    c = [[X[m] * xr[n][a] - X[n] * xr[m][a] for a in range(3)] for (m, n) in cyc]
+    # xi_a x_l: the (a+1, a+2) cyclic pair of reference derivatives of c[l]
+    xix = [[None] * 3 for _ in range(3)]  # xix[a][l] = d xi_a / d x_l
+    for l in range(3):
+        for a in range(3):
+            b, g = (a + 1) % 3, (a + 2) % 3
+            xix[a][l] = (_d_along(D, c[l][g], b) - _d_along(D, c[l][b], g)) * JI2
+    # the x_xi fields are then overwritten with (1/det) A A^T, A the adjugate of the
+    # discrete metric, as computemetric! does (Metrics.jl:606-666); that is not the
+    # inverse Jacobian (on a unit brick it gives 2, not 1/2), and no DG kernel reads it
+    A = [[None] * 3 for _ in range(3)]
+    A[0][0] = xix[1][1] * xix[2][2] - xix[1][2] * xix[2][1]
+    A[0][1] = xix[0][2] * xix[2][1] - xix[0][1] * xix[2][2]
+    A[0][2] = xix[0][1] * xix[1][2] - xix[0][2] * xix[1][1]
+    A[1][0] = xix[1][2] * xix[2][0] - xix[1][0] * xix[2][2]
+    A[1][1] = xix[0][0] * xix[2][2] - xix[0][2] * xix[2][0]
+    A[1][2] = xix[0][2] * xix[1][0] - xix[0][0] * xix[1][2]
+    A[2][0] = xix[1][0] * xix[2][1] - xix[1][1] * xix[2][0]
+    A[2][1] = xix[0][1] * xix[2][0] - xix[0][0] * xix[2][1]
+    A[2][2] = xix[0][0] * xix[1][1] - xix[0][1] * xix[1][0]
+    det = xix[0][0] * A[0][0] + xix[1][0] * A[0][1] + xix[2][0] * A[0][2]
+    xinv = [[(1.0 / det) * (A[d][0] * A[a][0] + A[d][1] * A[a][1] + A[d][2] * A[a][2]) for a in range(3)]
+            for d in range(3)]  # xinv[d][a] = x_d xi_a
+    W = np.empty(Np)
+    ii, jj, kk = np.arange(Np) % Nq, (np.arange(Np) // Nq) % Nq, np.arange(Np) // (Nq * Nq)
+    W[:] = (w[kk] * w[jj]) * w[ii]
+    flat = lambda f: f.reshape(Np, E, order="F")
+    vgeo = np.zeros((Np, NVGEO, E), order="F")
+    for l in range(3):
+        for a in range(3):
+            vgeo[:, a + 3 * l, :] = flat(xix[a][l])
+    Jf = flat(J)
+    vgeo[:, M_ID, :] = Jf * W[:, None]
+    vgeo[:, MI_ID, :] = 1.0 / vgeo[:, M_ID, :]
+    MH = (w[jj] * w[ii])[:, None]
+    vgeo[:, MH_ID, :] = MH * np.sqrt((Jf * flat(xix[2][0])) ** 2 + (Jf * flat(xix[2][1])) ** 2
+                                     + (Jf * flat(xix[2][2])) ** 2)
+    for d in range(3):
+        vgeo[:, X1_ID + d, :] = x[d]
+    vgeo[:, JCV_ID, :] = flat(JcV)
+    for a in range(3):
+        for d in range(3):
+            vgeo[:, 16 + d + 3 * a, :] = flat(xinv[d][a])
+    # surface: n = -/+ J d xi_a / d x on faces 2a+1 / 2a+2, sJ = |n|, n /= sJ
+    a_, b_ = np.arange(Nfp) % Nq, np.arange(Nfp) // Nq
+    sgeo = np.zeros((5, Nfp, 6, E), order="F")
+    for f in range(6):
+        ax = f // 2
+        sgn = -1.0 if f % 2 == 0 else 1.0
+        vid = fmask[f]
+        nv = [sgn * Jf[vid] * vgeo[vid, ax + 3 * l, :] for l in range(3)]
+        sJ = np.sqrt(nv[0] ** 2 + nv[1] ** 2 + nv[2] ** 2)
+        for l in range(3):
+            sgeo[N1_ID + l, :, f, :] = nv[l] / sJ
+        sgeo[SM_ID, :, f, :] = sJ * (w[b_] * w[a_])[:, None]
+        sgeo[VMI_ID, :, f, :] = vgeo[vid, MI_ID, :]
+    return vgeo, sgeo
+
+
 class BrickGrid:
-    """Axis-aligned brick DG grid.
+    """Brick-topology DG grid, axis-aligned unless warped.
 
     ``brickrange``: three 1-D arrays of element break points (like the
     reference's ``brickrange`` tuple); ``periodicity``: three bools;
     ``boundary``: ((xmin_tag, xmax_tag), (ymin, ymax), (zmin, zmax)) tags used on
-    non-periodic faces (AtmosLESConfiguration default ((0,0),(0,0),(1,2))).
+    non-periodic faces (AtmosLESConfiguration default ((0,0),(0,0),(1,2)));
+    ``meshwarp``: optional vectorised map (x1, x2, x3) -> (x1', x2', x3') applied to
+    the node coordinates, as the reference grid's ``meshwarp`` keyword.  A warp
+    must map periodic faces onto their images shifted by the period.
     """
 
     def __init__(self, brickrange, N: int = 4, periodicity=(True, True, True),
-                 boundary=((1, 1), (1, 1), (1, 2))):
+                 boundary=((1, 1), (1, 1), (1, 2)), meshwarp=None):
         self.N = N
         self.Nq = Nq = N + 1
         self.Np = Np = Nq ** 3
@@ -201,6 +304,17 @@ class BrickGrid:
             sgeo[SM_ID, :, f, :] = sJ[d][None, :] * sw[d][:, None]
             sgeo[VMI_ID, :, f, :] = vgeo[self.fmask[f], MI_ID, :]
         self.sgeo = sgeo
+        self.warped = meshwarp is not None
+        if self.warped:
+            # creategrid! (Metrics.jl:85-120): trilinear blend of the element's corners,
+            # then the warp of every node (Grids.jl:1049-1060), then the metrics
+            xs = []
+            for d, (l0, l1) in enumerate(((x0, x0 + h1), (y0, y0 + h2), (z0, z0 + h3))):
+                t = r[(ii, jj, kk)[d]][:, None]
+                xs.append(((1 - t) * l0[None, :] + (1 + t) * l1[None, :]) / 2)
+            xw = meshwarp(*xs)
+            xw = np.stack([np.broadcast_to(np.asarray(c, dtype=np.float64), xs[0].shape) for c in xw])
+            self.vgeo, self.sgeo = curvilinear_geometry(xw, self.D, w, self.fmask)
 
         # connectivity
         vmm = np.empty((Nfp, 6, E), dtype=np.int64, order="F")
@@ -249,13 +363,24 @@ class BrickGrid:
         return self.Np * self.nelem
 
     def min_node_distance(self) -> float:
-        """Smallest LGL node gap over all elements and directions."""
-        gaps = np.diff(self.xi).min() / 2
-        return float(min(np.min(np.diff(b)) for b in self.brickrange) * gaps)
+        """Smallest distance between neighbouring nodes of an element along a
+        reference direction (the distance of kernel_min_neighbor_distance!,
+        Grids.jl:1228-1334); the smallest LGL gap times the smallest element edge on
+        an unwarped brick."""
+        if not self.warped:
+            gaps = np.diff(self.xi).min() / 2
+            return float(min(np.min(np.diff(b)) for b in self.brickrange) * gaps)
+        Nq, E = self.Nq, self.nelem
+        X = np.stack([self.vgeo[:, X1_ID + d, :].reshape(Nq, Nq, Nq, E, order="F") for d in range(3)])
+        best = np.inf
+        for a in range(3):
+            d = np.diff(X, axis=1 + a)
+            best = min(best, float(np.sqrt((d ** 2).sum(axis=0)).min()))
+        return best
 
 
 def uniform_brick(n, lo, hi, N=4, periodicity=(True, True, True),
-                  boundary=((1, 1), (1, 1), (1, 2))) -> BrickGrid:
-    """Brick of n = (nx, ny, nz) equal elements on [lo, hi]."""
+                  boundary=((1, 1), (1, 1), (1, 2)), meshwarp=None) -> BrickGrid:
+    """Brick of n = (nx, ny, nz) equal elements on [lo, hi] (optionally warped)."""
     br = [np.linspace(lo[d], hi[d], n[d] + 1) for d in range(3)]
-    return BrickGrid(br, N=N, periodicity=periodicity, boundary=boundary)
+    return BrickGrid(br, N=N, periodicity=periodicity, boundary=boundary, meshwarp=meshwarp)

@@ -524,17 +524,10 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
 
     const int Np = c->Np, Nfp = c->Nfp;
     const int64_t E = c->nelem;
-    // geometry: xi (9), M, MI per node
-    std::vector<double> vg((size_t)Np * NVG * E);
-    for (int64_t e = 0; e < E; ++e)
-        for (int f = 0; f < NVG; ++f)
-            std::memcpy(&vg[(size_t)Np * (f + NVG * e)], d->vgeo + (size_t)Np * (f + (int64_t)d->nvgeo * e), Np * sizeof(double));
     // connectivity: minus side must be the element's own face nodes (Grids.jl:615)
-    std::vector<int32_t> nbr((size_t)Nfp * 6 * E), bnd((size_t)6 * E);
     const int Nq = c->Nq, NN = c->N;
     for (int64_t e = 0; e < E; ++e)
-        for (int f = 0; f < 6; ++f) {
-            bnd[f + 6 * e] = (int32_t)d->elemtobndy[f + 6 * e];
+        for (int f = 0; f < 6; ++f)
             for (int fn = 0; fn < Nfp; ++fn) {
                 int a = fn % Nq, b = fn / Nq, vid;
                 switch (f) {
@@ -550,23 +543,13 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
                     return fail(c, CMDG_EINVAL, "vmap_minus is not the element's own face nodes");
                 const int64_t p = d->vmap_plus[id] - 1;
                 if (p < 0 || p >= (int64_t)Np * E) return fail(c, CMDG_EINVAL, "vmap_plus out of range");
-                nbr[id] = (int32_t)p;
             }
-        }
     int rc;
-    if ((rc = upload(c, &c->d_vg, vg.data(), vg.size()))) return rc;
-    if ((rc = upload(c, &c->d_sg, d->sgeo, (size_t)5 * Nfp * 6 * E))) return rc;
-    if ((rc = upload(c, &c->d_nbr, nbr.data(), nbr.size()))) return rc;
-    if ((rc = upload(c, &c->d_bnd, bnd.data(), bnd.size()))) return rc;
     if ((rc = upload(c, &c->d_D, d->D, (size_t)Nq * Nq))) return rc;
     if ((rc = upload(c, &c->d_aux, d->aux_init, (size_t)Np * (d->naux > 0 ? d->naux : 1) * E))) return rc;
     if ((rc = upload<double>(c, &c->d_gf, nullptr, (size_t)Np * c->ngf * E))) return rc;
     if ((rc = upload<double>(c, &c->d_Qs, nullptr, (size_t)Np * 5 * E))) return rc;
     if ((rc = check_hip(c, hipMemset(c->d_gf, 0, (size_t)Np * c->ngf * E * sizeof(double)), "hipMemset"))) return rc;
-    c->A.vg = c->d_vg;
-    c->A.sg = c->d_sg;
-    c->A.nbr = c->d_nbr;
-    c->A.bnd = c->d_bnd;
     c->A.D = c->d_D;
     c->A.nelem = E;
     std::vector<int64_t> crep;  // first element of each static-field class (empty: no classes)
@@ -656,6 +639,26 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
                 if (passes == "both" || passes == "tendency") c->ord_tend = c->d_order;
             }
         }
+    }
+    if (!c->affine) {
+        // general meshes: per-node geometry xi (9), M, MI, the face geometry and the
+        // neighbour node of every face node (affine bricks read none of these)
+        std::vector<double> vg((size_t)Np * NVG * E);
+        for (int64_t e = 0; e < E; ++e)
+            for (int f = 0; f < NVG; ++f)
+                std::memcpy(&vg[(size_t)Np * (f + NVG * e)], d->vgeo + (size_t)Np * (f + (int64_t)d->nvgeo * e),
+                            Np * sizeof(double));
+        std::vector<int32_t> nbr((size_t)Nfp * 6 * E), bnd((size_t)6 * E);
+        for (int64_t i = 0; i < (int64_t)Nfp * 6 * E; ++i) nbr[i] = (int32_t)(d->vmap_plus[i] - 1);
+        for (int64_t i = 0; i < 6 * E; ++i) bnd[i] = (int32_t)d->elemtobndy[i];
+        if ((rc = upload(c, &c->d_vg, vg.data(), vg.size()))) return rc;
+        if ((rc = upload(c, &c->d_sg, d->sgeo, (size_t)5 * Nfp * 6 * E))) return rc;
+        if ((rc = upload(c, &c->d_nbr, nbr.data(), nbr.size()))) return rc;
+        if ((rc = upload(c, &c->d_bnd, bnd.data(), bnd.size()))) return rc;
+        c->A.vg = c->d_vg;
+        c->A.sg = c->d_sg;
+        c->A.nbr = c->d_nbr;
+        c->A.bnd = c->d_bnd;
     }
     if (d->precision == CMDG_MIXED) {
         if (!c->affine)
@@ -1011,8 +1014,10 @@ int cmdg_weightedsum(cmdg_ctx *c, const double *Q, int32_t state, double *out) {
     if (nb > 1024) nb = 1024;
     int rc = diag_scratch(c, nb);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_weightedsum, dim3((unsigned)nb), dim3(256), 0, c->stream, Q, c->d_vg, c->Np, NVG, state,
-                       c->nelem, c->d_part);
+    // M per node: vgeo's mass on general meshes; J w_i w_j w_k on affine bricks, which
+    // the create-time check found equal to vgeo's M bit for bit
+    hipLaunchKernelGGL(k_weightedsum, dim3((unsigned)nb), dim3(256), 0, c->stream, Q, c->d_vg, c->Np, NVG,
+                       c->affine ? c->d_geo : nullptr, c->d_W, state, c->nelem, c->d_part);
     hipLaunchKernelGGL(k_dd_reduce, dim3(1), dim3(256), 0, c->stream, c->d_part, (int)nb, c->d_scalar);
     return diag_result(c, out);
 }

@@ -129,8 +129,11 @@ __device__ __forceinline__ DD dd_prod(double a, double b) {
 }
 
 // sum over the nodes of M * Q[:, state, :], M = the mass-matrix diagonal (vg field 9)
+// M per node from vg (per-node geometry, field 9) or, when geo is set, from the affine
+// element record: M = J * W[n] (J at geo[AFF_GEO e + 3]).
 __global__ void __launch_bounds__(256) k_weightedsum(const double *__restrict__ Q, const double *__restrict__ vg,
-                                                     int Np, int nvg, int state, int64_t nelem,
+                                                     int Np, int nvg, const double *__restrict__ geo,
+                                                     const double *__restrict__ W, int state, int64_t nelem,
                                                      double2 *__restrict__ partial) {
     __shared__ DD red[256];
     DD acc{0.0, 0.0};
@@ -138,7 +141,8 @@ __global__ void __launch_bounds__(256) k_weightedsum(const double *__restrict__ 
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (int64_t)gridDim.x * 256) {
         const int64_t e = t / Np;
         const int n = (int)(t - e * Np);
-        acc = dd_add(acc, dd_prod(Q[(int64_t)Np * (state + 5 * e) + n], vg[(int64_t)Np * (9 + (int64_t)nvg * e) + n]));
+        const double M = geo ? geo[(int64_t)AFF_GEO * e + 3] * W[n] : vg[(int64_t)Np * (9 + (int64_t)nvg * e) + n];
+        acc = dd_add(acc, dd_prod(Q[(int64_t)Np * (state + 5 * e) + n], M));
     }
     red[threadIdx.x] = acc;
     __syncthreads();

@@ -58,6 +58,14 @@ template <class T> struct Ctx {
     T R_d, cp_d, cv_d, T_0, grav, MSLP, visc, C_smag, inv_Pr;
     // learned closure: BF16-valued weights held as double
     double w1[320], b1[32], w2[1024], b2[32], w3[32], b3, shift[10], scale[10];
+    // test set-ups of the reference's manufactured-solution test
+    // (test/Numerics/DGMethods/compressible_Navier_Stokes/mms_bc_atmos.jl):
+    // qbnd, when set, is the state InitStateBC prescribes at every node (Np, 5, nelem),
+    // read at the nodes of boundary faces; src, when set, a per-node source
+    // (Np, 5, nelem) added like source! in the vertical volume pass; htot_zero
+    // replaces total_specific_enthalpy by zero (mms_bc_atmos.jl:48-49).
+    const double *qbnd = nullptr, *src = nullptr;
+    bool htot_zero = false;
 };
 
 // Round to the nearest BF16 value (ties to even), as the GPU's conversion does.
@@ -233,7 +241,7 @@ template <class T> inline void gradient_argument(const Ctx<T> &c, const T *q, co
     G[2] = rinv * q[3];
     TS<T> ts = thermo(c, q, a.Phi);
     T e_tot = q[4] * (T(1) / q[0]);
-    G[3] = e_tot + c.R_d * ts.temp;
+    G[3] = c.htot_zero ? T(0) : e_tot + c.R_d * ts.temp;
     G[4] = c.smag ? a.theta_v : T(0);
 }
 
@@ -373,6 +381,15 @@ template <class T> void interface_gradients(Ctx<T> &c, const T *Q, const T *aux)
                     gradient_argument(c, qp, ap, Gp);
                     for (int d = 0; d < 3; ++d)
                         for (int s = 0; s < ng; ++s) Gs[d][s] = nv[d] * (Gp[s] + Gm[s]) / T(2);
+                } else if (c.qbnd) {
+                    // InitStateBC (bc_initstate.jl:12-27) through the generic boundary
+                    // gradient flux (NumericalFluxes.jl:85-123): G* = n G(q+), q+ the
+                    // prescribed state, no aux refresh (it is not an AtmosBC)
+                    T qb[5];
+                    for (int s = 0; s < 5; ++s) qb[s] = (T)c.qbnd[x.vm + (int64_t)Np * (s + 5 * x.em)];
+                    gradient_argument(c, qb, am, Gp);
+                    for (int d = 0; d < 3; ++d)
+                        for (int s = 0; s < ng; ++s) Gs[d][s] = nv[d] * Gp[s];
                 } else {
                     // boundary_state! then the moisture aux refresh of the + side
                     // (boundaryconditions.jl:58-82), then G* = n G(q+).
@@ -442,6 +459,8 @@ template <class T> void volume_tendency(Ctx<T> &c, T *tend, const T *Q, const T 
                 for (int s = 0; s < 5; ++s) Ft[n][ax][s] = M * ((x1 * F[0][s] + x2 * F[1][s]) + x3 * F[2][s]);
             }
             for (int s = 0; s < 5; ++s) src[n][s] = T(0);
+            if (c.src)
+                for (int s = 0; s < 5; ++s) src[n][s] = -T(0) + (T)c.src[n + (int64_t)Np * (s + 5 * e)];
             if (c.d.gravity_source) {
                 // Gravity (tendencies_momentum.jl:60-72)
                 T drho = (c.d.ref_state == CMDG_HYDROSTATIC_STATE && c.d.subtract_off) ? (q[0] - a.ref_rho) : q[0];
@@ -494,6 +513,12 @@ inline void numerical_flux(const Ctx<T> &c, const T *nv, const T *qm, const Node
         flux_second_order(c, qp, gfp, ap, Fp);
         for (int s = 0; s < 5; ++s)
             flux[s] += ((Fm[0][s] + Fp[0][s]) * h[0] + (Fm[1][s] + Fp[1][s]) * h[1]) + (Fm[2][s] + Fp[2][s]) * h[2];
+    } else if (c.diffusive && c.qbnd) {
+        // InitStateBC: boundary_flux_second_order! (NumericalFluxes.jl:919-967), the
+        // second-order flux of the prescribed + state with the + gradient flux (the
+        // node's own, vmap+ = vmap- on a boundary face), dotted with n
+        flux_second_order(c, qp, gfp, ap, Fp);
+        for (int s = 0; s < 5; ++s) flux[s] += ((Fp[0][s] * nv[0] + Fp[1][s] * nv[1]) + Fp[2][s] * nv[2]);
     }
 }
 
@@ -524,7 +549,10 @@ template <class T> void interface_tendency(Ctx<T> &c, T *tend, const T *Q, const
                     numerical_flux(c, nv, qm, am, gfm, qp, ap, gfp, false, flux);
                 } else {
                     T qb[5];
-                    wall_state(qm, nv, false, qb);
+                    if (c.qbnd)  // InitStateBC: the prescribed state (bc_initstate.jl:12-27)
+                        for (int s = 0; s < 5; ++s) qb[s] = (T)c.qbnd[x.vm + (int64_t)Np * (s + 5 * x.em)];
+                    else
+                        wall_state(qm, nv, false, qb);
                     numerical_flux(c, nv, qm, am, gfm, qb, am, gfm, true, flux);
                 }
                 for (int s = 0; s < 5; ++s) tend[x.vm + (int64_t)Np * (s + 5 * x.em)] -= alpha * vMI * sM * flux[s];
@@ -873,6 +901,18 @@ int oracle_mlp_forward(oracle_ctx *o, const double *raw, int64_t n, double *y) {
     if (o->use_float) return 1;
     Ctx<double> &c = *(Ctx<double> *)o->impl;
     for (int64_t i = 0; i < n; ++i) y[i] = mlp_output(c, raw + 10 * i);
+    return 0;
+}
+
+// Manufactured-solution set-up (double contexts): qbnd / src as described at Ctx
+// (either may be null; the arrays are read during later oracle_rhs calls and stay
+// owned by the caller), htot_zero != 0 sets total_specific_enthalpy to zero.
+int oracle_set_prescribed(oracle_ctx *o, const double *qbnd, const double *src, int htot_zero) {
+    if (o->use_float) return 1;
+    Ctx<double> &c = *(Ctx<double> *)o->impl;
+    c.qbnd = qbnd;
+    c.src = src;
+    c.htot_zero = htot_zero != 0;
     return 0;
 }
 

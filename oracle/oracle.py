@@ -22,15 +22,17 @@ from cmdg.atmos import AtmosConfig  # noqa: E402
 from cmdg.grid import BrickGrid  # noqa: E402
 
 LIB_PATH = os.path.join(HERE, "lib", "liboracle.so")
-_lib = None
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
-        lib = C.CDLL(LIB_PATH)
+def load(path: str = LIB_PATH):
+    """The oracle library at `path` (default: the built oracle) with its argument
+    types declared; other paths are test builds of modified oracle sources."""
+    lib = _libs.get(path)
+    if lib is None:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
+        lib = C.CDLL(path)
         vp = C.c_void_p
         lib.oracle_create.argtypes = [C.POINTER(abi.CmdgDesc), C.c_int]
         lib.oracle_create.restype = vp
@@ -50,8 +52,9 @@ def load():
         lib.oracle_courant.restype = C.c_double
         lib.oracle_weightedsum.argtypes = [vp, vp, C.c_int]
         lib.oracle_weightedsum.restype = C.c_double
-        _lib = lib
-    return _lib
+        lib.oracle_set_prescribed.argtypes = [vp, vp, vp, C.c_int]
+        _libs[path] = lib
+    return lib
 
 
 def _p(a: np.ndarray):
@@ -62,8 +65,9 @@ def _p(a: np.ndarray):
 class Oracle:
     """Oracle DG model on ``grid`` with the options of ``cfg``."""
 
-    def __init__(self, cfg: AtmosConfig, grid: BrickGrid, aux: np.ndarray, use_float: bool = False):
-        self.lib = load()
+    def __init__(self, cfg: AtmosConfig, grid: BrickGrid, aux: np.ndarray, use_float: bool = False,
+                 lib_path: str = LIB_PATH):
+        self.lib = load(lib_path)
         self.cfg, self.grid = cfg, grid
         self.dtype = np.float32 if use_float else np.float64
         desc, keep = cfg.descriptor(grid, np.asfortranarray(aux, dtype=np.float64))
@@ -135,6 +139,16 @@ class Oracle:
     def weightedsum(self, Q, state: int) -> float:
         Qf = np.asfortranarray(Q, dtype=np.float64)
         return self.lib.oracle_weightedsum(self._ctx, _p(Qf), int(state))
+
+    def set_prescribed(self, qbnd=None, src=None, htot_zero=False):
+        """Manufactured-solution set-up: qbnd (Np, 5, E), the state InitStateBC
+        prescribes (read at boundary-face nodes), src (Np, 5, E) a per-node source,
+        htot_zero the zero total specific enthalpy override.  The arrays are kept
+        referenced here and read by later rhs calls."""
+        self._prescribed = [None if a is None else np.asfortranarray(a, dtype=np.float64) for a in (qbnd, src)]
+        qb, sr = self._prescribed
+        self.lib.oracle_set_prescribed(self._ctx, None if qb is None else _p(qb), None if sr is None else _p(sr),
+                                       int(bool(htot_zero)))
 
     def set_mlp(self, mlp):
         """Load a closure.MLPParams network (weights rounded to BF16 inside)."""
