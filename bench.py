@@ -54,9 +54,11 @@ TRAIN_META = os.path.join(ROOT, "climatemachine.jl_amd", "cmdg", "data", "tgv_cl
 COUNTS = os.path.join(ROOT, "climatemachine.jl_amd", "cmdg", "data", "algorithmic_counts.json")
 # counters of the two passes (rocprofv3 collects at most 8 SQ and 4 TCC counters a run)
 PASSES = (("FETCH_SIZE", "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_FMA_F32",
-           "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_MFMA_MOPS_BF16", "SQ_INSTS_VALU", "SQ_WAVES"),
+           "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_MFMA_MOPS_BF16", "SQ_INSTS_VALU_FLOPS_FP32",
+           "SQ_INSTS_VALU_FLOPS_FP64", "SQ_WAVES"),
           ("WRITE_SIZE", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
-           "SQ_INSTS_VALU_TRANS_F64", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES"))
+           "SQ_INSTS_VALU_TRANS_F64", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "SQ_INSTS_VALU",
+           "SQ_INSTS_VALU_FLOPS_FP32_TRANS"))
 
 
 def build_les(ne=25):
@@ -86,11 +88,13 @@ def tgv_viscosity():
         return float(json.load(f)["nu"])
 
 
-def build_tgv(ne=96):
+def build_tgv(ne=96, lean=True):
+    """The Taylor-Green workload; lean: the brick without per-node geometry arrays (the
+    library builds its element records from the break points, cmdg_create_brick)."""
     from cmdg import abi, atmos
     from cmdg.grid import uniform_brick
     prm = atmos.ideal_gas_parameters()
-    grid = uniform_brick((ne, ne, ne), (0.0, 0.0, 0.0), (2 * math.pi,) * 3, N=4)
+    grid = uniform_brick((ne, ne, ne), (0.0, 0.0, 0.0), (2 * math.pi,) * 3, N=4, lean=lean)
     cfgs = {"plain": atmos.AtmosConfig(prm, turbulence=abi.SMAGORINSKY_LILLY, C_smag=0.21),
             "learned": atmos.AtmosConfig(prm, turbulence=abi.LEARNED_CLOSURE, C_smag=0.21,
                                          viscosity=tgv_viscosity())}
@@ -134,7 +138,7 @@ def cpu_baseline(workload, steps):
     import numpy as np
     from oracle import oracle as O
     if workload == "tgv96":
-        grid, cfgs, aux, Q0, dt = build_tgv(16)
+        grid, cfgs, aux, Q0, dt = build_tgv(16, lean=False)   # the oracle reads per-node geometry
         label = (f"oracle, plain FP64 Smagorinsky-Lilly, {steps} LSRK54 steps (5 RHS stages each) of a "
                  f"16^3-element sample of the Taylor-Green workload ({grid.Np * grid.nelem:,} nodes) after 1 "
                  "warm-up step; per-DOF rate extrapolated to the 96^3 workload")
@@ -293,7 +297,12 @@ def flop_rates(cnt, avg_ms):
     f32, f64 = per("F32"), per("F64")
     mfma = 512 * cnt.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0)
     s = avg_ms * 1e-3
-    return {"fp32_tflops": round(f32 / s / 1e12, 3), "fp64_tflops": round(f64 / s / 1e12, 3),
+    direct = {}
+    for k, name in (("FP32", "SQ_INSTS_VALU_FLOPS_FP32"), ("FP64", "SQ_INSTS_VALU_FLOPS_FP64"),
+                    ("FP32_TRANS", "SQ_INSTS_VALU_FLOPS_FP32_TRANS")):
+        if name in cnt:
+            direct[k.lower() + "_tflops_flops_counter"] = round(cnt[name] / s / 1e12, 3)
+    return {**direct, "fp32_tflops": round(f32 / s / 1e12, 3), "fp64_tflops": round(f64 / s / 1e12, 3),
             "bf16_mfma_tflops": round(mfma / s / 1e12, 3),
             "fp32_frac": round(f32 / s / 1e12 / PEAK_TFLOPS["fp32"], 4),
             "fp64_frac": round(f64 / s / 1e12 / PEAK_TFLOPS["fp64"], 4),

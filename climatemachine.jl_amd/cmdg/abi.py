@@ -8,6 +8,8 @@ from __future__ import annotations
 import ctypes as C
 import os
 
+import numpy as np
+
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 ROOT_DIR = os.path.dirname(PKG_DIR)          # climatemachine.jl_amd/
 LIB_PATH = os.path.join(ROOT_DIR, "lib", "libcmdg_hip.so")
@@ -66,11 +68,11 @@ class CmdgDesc(C.Structure):
 
 
 EXPORTED = (
-    "cmdg_create", "cmdg_destroy", "cmdg_rhs", "cmdg_lsrk54_step", "cmdg_lsrk54_steps",
+    "cmdg_create", "cmdg_create_brick", "cmdg_destroy", "cmdg_rhs", "cmdg_lsrk54_step", "cmdg_lsrk54_steps",
     "cmdg_lsrk_stage", "cmdg_lsrk_update", "cmdg_aux", "cmdg_gradient_flux",
     "cmdg_gradient_flux_f32", "cmdg_mlp_set_weights", "cmdg_mlp_forward", "cmdg_velocity_gradient",
     "cmdg_box_filter", "cmdg_closure_samples", "cmdg_mlp_loss_grad", "cmdg_mlp_adam", "cmdg_interpolate_equispaced",
-    "cmdg_courant", "cmdg_weightedsum", "cmdg_geometry_kind", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
+    "cmdg_courant", "cmdg_weightedsum", "cmdg_not_finite_fields", "cmdg_geometry_kind", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
     "cmdg_last_error", "cmdg_version",
 )
 MLP_NPARAM = 1441
@@ -93,6 +95,7 @@ def load_library(path: str = None):
     lib = C.CDLL(path)
     vp = C.c_void_p
     lib.cmdg_create.argtypes = [C.POINTER(CmdgDesc), vp, C.POINTER(vp)]
+    lib.cmdg_create_brick.argtypes = [C.POINTER(CmdgDesc), C.POINTER(CmdgBrick), vp, C.POINTER(vp)]
     lib.cmdg_destroy.argtypes = [vp]
     lib.cmdg_rhs.argtypes = [vp, vp, vp, C.c_double, C.c_double, C.c_double]
     lib.cmdg_lsrk54_step.argtypes = [vp, vp, vp, C.c_double, C.c_double]
@@ -115,6 +118,7 @@ def load_library(path: str = None):
                                                 C.c_int64, vp, vp]
     lib.cmdg_courant.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_int32, C.POINTER(C.c_double)]
     lib.cmdg_weightedsum.argtypes = [vp, vp, C.c_int32, C.POINTER(C.c_double)]
+    lib.cmdg_not_finite_fields.argtypes = [vp, vp, C.POINTER(C.c_int32)]
     lib.cmdg_geometry_kind.argtypes = [vp, C.POINTER(C.c_int32)]
     lib.cmdg_set_kernel_timing.argtypes = [vp, C.c_int]
     lib.cmdg_kernel_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int64), C.c_int32]
@@ -131,8 +135,25 @@ def load_library(path: str = None):
 
 
 def as_dp(a):
-    return a.ctypes.data_as(_dp)
+    return None if a is None else a.ctypes.data_as(_dp)
 
 
 def as_ip(a):
-    return a.ctypes.data_as(_ip)
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+class CmdgBrick(C.Structure):
+    """cmdg_brick (include/cmdg.h)."""
+    _fields_ = [("ne", C.c_int64 * 3), ("x", _dp * 3), ("periodic", C.c_int32 * 3), ("tag", (C.c_int64 * 2) * 3)]
+
+
+def brick_descriptor(grid):
+    """cmdg_brick of a BrickGrid; the returned list keeps the break-point arrays alive."""
+    keep = [np.ascontiguousarray(b, dtype=np.float64) for b in grid.brickrange]
+    b = CmdgBrick()
+    for a in range(3):
+        b.ne[a] = len(keep[a]) - 1
+        b.x[a] = keep[a].ctypes.data_as(_dp)
+        b.periodic[a] = int(bool(grid.periodicity[a]))
+        b.tag[a][0], b.tag[a][1] = int(grid.boundary[a][0]), int(grid.boundary[a][1])
+    return b, keep

@@ -154,16 +154,17 @@ class AtmosConfig:
         return 10 if self.turbulence == abi.SMAGORINSKY_LILLY else 9
 
     def descriptor(self, grid: BrickGrid, aux: np.ndarray, precision: int = abi.FP64):
-        """cmdg_desc for this model on ``grid``; the returned tuple keeps the arrays alive."""
+        """cmdg_desc for this model on ``grid``; the returned tuple keeps the arrays alive.
+        On a lean grid the per-node geometry fields are null (cmdg_create_brick)."""
         lay = self.layout
         g = lambda k: lay.get(k, -1)
         prm = self.params
-        keep = [np.asfortranarray(a) for a in (grid.vgeo, grid.sgeo, grid.vmap_minus, grid.vmap_plus,
-                                                grid.elemtobndy, grid.D, grid.omega, aux)]
+        per_node = (grid.vgeo, grid.sgeo, grid.vmap_minus, grid.vmap_plus, grid.elemtobndy)
+        keep = [None if a is None else np.asfortranarray(a) for a in per_node + (grid.D, grid.omega, aux)]
         vgeo, sgeo, vmm, vmp, bnd, D, om, auxf = keep
         d = abi.CmdgDesc()
         d.N = grid.N
-        d.nvgeo = vgeo.shape[1]
+        d.nvgeo = 0 if vgeo is None else vgeo.shape[1]
         d.nelem = grid.nelem
         d.naux = self.naux
         d.aux_coord = g("coord")
@@ -253,6 +254,11 @@ def pressure_gradient(grid: BrickGrid, p: np.ndarray) -> np.ndarray:
 
 def lengthscale(grid: BrickGrid) -> np.ndarray:
     """2 / (cbrt(det(d xi / d x)) N)  (Geometry.jl:121-122), (Np, E)."""
+    if grid.lean:
+        # the per-node formula below on diag(2/h): det = a00 (a11 a22), the same bits
+        h1, h2, h3 = grid.element_sizes
+        det = (2.0 / h1) * ((2.0 / h2) * (2.0 / h3))
+        return np.broadcast_to((2.0 / (np.cbrt(det) * max(1, grid.N)))[None, :], (grid.Np, grid.nelem))
     v = grid.vgeo
     a = [[v[:, i + 3 * j, :] for j in range(3)] for i in range(3)]
     det = (a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1])
@@ -269,6 +275,8 @@ def init_aux(cfg: AtmosConfig, grid: BrickGrid) -> np.ndarray:
     aux[:, lay["coord"] + 0, :] = x1
     aux[:, lay["coord"] + 1, :] = x2
     aux[:, lay["coord"] + 2, :] = x3
+    if grid.lean and (cfg.orientation == abi.FLAT_ORIENTATION or cfg.ref_profile is not None):
+        raise ValueError("a lean grid has no face geometry for the DG gradient of Phi and p_ref")
     if cfg.orientation == abi.FLAT_ORIENTATION:
         Phi = prm.grav * x3
         aux[:, lay["Phi"], :] = Phi

@@ -9,7 +9,9 @@ contiguous IEEE float64 little-endian dataset at the root named as the Julia
 variable, with the dimensions listed in reverse (Julia arrays are column-major, HDF5
 dataspaces row-major, so the bytes are the Julia array's own), `t` is a scalar
 dataset, and the first 512 bytes (the HDF5 user block) hold JLD2's file-header
-string.  read_checkpoint returns (h_Q, h_aux, t) as Fortran-ordered NumPy arrays
+string.  The file format objects are those of HDF5 1.8 (superblock version 2,
+version-2 object headers), the layout JLD2 writes, rather than the library's
+earliest-compatible default (superblock version 0).  read_checkpoint returns (h_Q, h_aux, t) as Fortran-ordered NumPy arrays
 (Checkpoint.jl:103-125), ready for DGModel(cfg, grid, aux=h_aux) and to_device(h_Q).
 
 The HDF5 library itself is the system libhdf5 (1.10, C API through ctypes):
@@ -33,6 +35,7 @@ _lib = None
 _hid = C.c_int64          # hid_t (HDF5 >= 1.10)
 _hsize = C.c_ulonglong    # hsize_t
 H5F_ACC_RDONLY, H5F_ACC_TRUNC = 0x0000, 0x0002
+H5F_LIBVER_V18 = 1        # H5F_libver_t: file format objects of HDF5 1.8
 H5P_DEFAULT, H5S_ALL, H5S_SCALAR = 0, 0, 0
 
 
@@ -56,6 +59,7 @@ def _load():
         raise OSError("H5open failed")
     for name, res, args in [
         ("H5Pcreate", _hid, [_hid]), ("H5Pset_userblock", C.c_int, [_hid, _hsize]),
+        ("H5Pset_libver_bounds", C.c_int, [_hid, C.c_int, C.c_int]),
         ("H5Pclose", C.c_int, [_hid]), ("H5Fcreate", _hid, [C.c_char_p, C.c_uint, _hid, _hid]),
         ("H5Fopen", _hid, [C.c_char_p, C.c_uint, _hid]), ("H5Fclose", C.c_int, [_hid]),
         ("H5Screate_simple", _hid, [C.c_int, C.POINTER(_hsize), C.POINTER(_hsize)]),
@@ -107,11 +111,14 @@ def write_checkpoint(Q, aux, t: float, checkpoint_dir: str, name: str, num: int,
     path = os.path.join(checkpoint_dir, checkpoint_filename(name, rank, num))
     arrays = {"h_Q": _host(Q), "h_aux": _host(aux)}
     fcpl = _check(lib.H5Pcreate(_global("H5P_CLS_FILE_CREATE_ID")), "H5Pcreate")
+    fapl = _check(lib.H5Pcreate(_global("H5P_CLS_FILE_ACCESS_ID")), "H5Pcreate")
     try:
         _check(lib.H5Pset_userblock(fcpl, USERBLOCK), "H5Pset_userblock")
-        fid = _check(lib.H5Fcreate(path.encode(), H5F_ACC_TRUNC, fcpl, H5P_DEFAULT), "H5Fcreate " + path)
+        _check(lib.H5Pset_libver_bounds(fapl, H5F_LIBVER_V18, H5F_LIBVER_V18), "H5Pset_libver_bounds")
+        fid = _check(lib.H5Fcreate(path.encode(), H5F_ACC_TRUNC, fcpl, fapl), "H5Fcreate " + path)
     finally:
         lib.H5Pclose(fcpl)
+        lib.H5Pclose(fapl)
     f64le, native = _global("H5T_IEEE_F64LE"), _global("H5T_NATIVE_DOUBLE")
     try:
         for key, a in arrays.items():

@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 from fractions import Fraction
 from typing import Callable, Optional, Sequence
 
@@ -87,7 +88,12 @@ class DGModel:
         with torch.cuda.device(self.device):
             stream = torch.cuda.current_stream(self.device).cuda_stream
             ctx = C.c_void_p()
-            rc = self.lib.cmdg_create(C.byref(desc), C.c_void_p(stream), C.byref(ctx))
+            if getattr(grid, "lean", False):
+                brick, bkeep = abi.brick_descriptor(grid)
+                rc = self.lib.cmdg_create_brick(C.byref(desc), C.byref(brick), C.c_void_p(stream), C.byref(ctx))
+                del bkeep
+            else:
+                rc = self.lib.cmdg_create(C.byref(desc), C.c_void_p(stream), C.byref(ctx))
         self._ctx = ctx
         if rc != abi.CMDG_OK:
             msg = self.lib.cmdg_last_error(ctx).decode() if ctx else "cmdg_create failed"
@@ -193,6 +199,21 @@ class DGModel:
         self._check(self.lib.cmdg_weightedsum(self._ctx, C.c_void_p(_ptr(Q)), int(state), C.byref(out)),
                     "cmdg_weightedsum")
         return out.value
+
+    def not_finite_fields(self, Q) -> list:
+        """Names of the state fields of Q that hold a NaN or Inf (the device map-reduce
+        of show_not_finite_fields, MPIStateArrays.jl:884-903)."""
+        m = C.c_int32()
+        self._check(self.lib.cmdg_not_finite_fields(self._ctx, C.c_void_p(_ptr(Q)), C.byref(m)),
+                    "cmdg_not_finite_fields")
+        return fields_of_mask(m.value)
+
+    def show_not_finite_fields(self, Q) -> list:
+        """Warn about (and return) the fields of Q that are not finite."""
+        bad = self.not_finite_fields(Q)
+        if bad:
+            warnings.warn(f"Field(s) ({', '.join(bad)}) are not finite (has NaNs or Inf)")
+        return bad
 
     @property
     def geometry_kind(self) -> str:
@@ -316,6 +337,34 @@ class LSRK54CarpenterKennedy:
             return
         for k in range(nsteps):
             self.dostep(Q, p, time + k * self.dt)
+
+
+STATE_FIELDS = ("ρ", "ρu[1]", "ρu[2]", "ρu[3]", "energy.ρe")
+
+
+def fields_of_mask(mask: int) -> list:
+    """Field names (AtmosModel prognostic order, AtmosModel.jl:387-403) of the set bits."""
+    return [name for s, name in enumerate(STATE_FIELDS) if mask >> s & 1]
+
+
+class NotFiniteCheck:
+    """solve callback that stops on a crash: every `every` steps it asks the library
+    which fields of Q are not finite and raises FloatingPointError naming them (the
+    reference stops through show_not_finite_fields and an exception,
+    MPIStateArrays.jl:884-927)."""
+
+    def __init__(self, dg, every: int = 1):
+        if every < 1:
+            raise ValueError("every must be >= 1")
+        self.dg, self.every = dg, every
+
+    def __call__(self, solver, Q, param, time):
+        if solver.steps % self.every:
+            return None
+        bad = self.dg.not_finite_fields(Q)
+        if bad:
+            raise FloatingPointError(f"step {solver.steps}, t = {time}: field(s) ({', '.join(bad)}) are not finite")
+        return None
 
 
 def solve(Q: torch.Tensor, solver, param=None, *, timeend: float = math.inf, adjustfinalstep: bool = True,

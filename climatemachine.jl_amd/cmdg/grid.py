@@ -219,10 +219,14 @@ class BrickGrid:
     ``meshwarp``: optional vectorised map (x1, x2, x3) -> (x1', x2', x3') applied to
     the node coordinates, as the reference grid's ``meshwarp`` keyword.  A warp
     must map periodic faces onto their images shifted by the period.
+    ``lean``: keep only the brick description (break points, periodicity, tags) and
+    per-element sizes, no per-node vgeo/sgeo/vmaps (the library builds its element
+    records from the description, cmdg_create_brick); coordinates and the mass matrix
+    are formed on demand.  For large meshes: a 96^3 brick's per-node arrays are ~30 GB.
     """
 
     def __init__(self, brickrange, N: int = 4, periodicity=(True, True, True),
-                 boundary=((1, 1), (1, 1), (1, 2)), meshwarp=None):
+                 boundary=((1, 1), (1, 1), (1, 2)), meshwarp=None, lean: bool = False):
         self.N = N
         self.Nq = Nq = N + 1
         self.Np = Np = Nq ** 3
@@ -270,6 +274,15 @@ class BrickGrid:
         kk = np.arange(Np) // (Nq * Nq)
         W[:] = (w[kk] * w[jj]) * w[ii]
         self.node_ijk = (ii, jj, kk)
+        self.lean = bool(lean)
+        self.warped = meshwarp is not None
+        self._h, self._x0, self._W = (h1, h2, h3), (x0, y0, z0), W
+        self._coords = None
+        if self.lean:
+            if self.warped:
+                raise ValueError("a lean grid is an unwarped brick")
+            self.vgeo = self.sgeo = self.vmap_minus = self.vmap_plus = self.elemtobndy = None
+            return
 
         vgeo = np.zeros((Np, NVGEO, E), order="F")
         J = (h1 / 2) * (h2 / 2) * (h3 / 2)
@@ -304,7 +317,6 @@ class BrickGrid:
             sgeo[SM_ID, :, f, :] = sJ[d][None, :] * sw[d][:, None]
             sgeo[VMI_ID, :, f, :] = vgeo[self.fmask[f], MI_ID, :]
         self.sgeo = sgeo
-        self.warped = meshwarp is not None
         if self.warped:
             # creategrid! (Metrics.jl:85-120): trilinear blend of the element's corners,
             # then the warp of every node (Grids.jl:1049-1060), then the metrics
@@ -351,12 +363,28 @@ class BrickGrid:
     @property
     def coords(self):
         """(Np, nelem) arrays x1, x2, x3."""
-        return (self.vgeo[:, X1_ID, :], self.vgeo[:, X2_ID, :], self.vgeo[:, X3_ID, :])
+        if not self.lean:
+            return (self.vgeo[:, X1_ID, :], self.vgeo[:, X2_ID, :], self.vgeo[:, X3_ID, :])
+        if self._coords is None:
+            r = self.xi
+            self._coords = tuple(np.asfortranarray(self._x0[d][None, :] + ((r[self.node_ijk[d]] + 1) / 2)[:, None]
+                                                   * self._h[d][None, :]) for d in range(3))
+        return self._coords
 
     @property
     def mass(self):
         """(Np, nelem) mass matrix: the MPIStateArray weights (create_states.jl:14-16)."""
-        return self.vgeo[:, M_ID, :]
+        if not self.lean:
+            return self.vgeo[:, M_ID, :]
+        h1, h2, h3 = self._h
+        return ((h1 / 2) * (h2 / 2) * (h3 / 2))[None, :] * self._W[:, None]
+
+    @property
+    def element_sizes(self):
+        """(h1, h2, h3): each element's edge lengths (unwarped bricks)."""
+        if self.warped:
+            raise ValueError("a warped grid has no element edge lengths")
+        return self._h
 
     @property
     def ndof_nodes(self) -> int:
@@ -380,7 +408,7 @@ class BrickGrid:
 
 
 def uniform_brick(n, lo, hi, N=4, periodicity=(True, True, True),
-                  boundary=((1, 1), (1, 1), (1, 2)), meshwarp=None) -> BrickGrid:
-    """Brick of n = (nx, ny, nz) equal elements on [lo, hi] (optionally warped)."""
+                  boundary=((1, 1), (1, 1), (1, 2)), meshwarp=None, lean: bool = False) -> BrickGrid:
+    """Brick of n = (nx, ny, nz) equal elements on [lo, hi] (optionally warped or lean)."""
     br = [np.linspace(lo[d], hi[d], n[d] + 1) for d in range(3)]
-    return BrickGrid(br, N=N, periodicity=periodicity, boundary=boundary, meshwarp=meshwarp)
+    return BrickGrid(br, N=N, periodicity=periodicity, boundary=boundary, meshwarp=meshwarp, lean=lean)

@@ -345,6 +345,9 @@ bool static_classes(const cmdg_desc *d, const KParams &P, int Np, std::vector<in
 // positions away.  The lattice position of each element comes from its centroid
 // (vgeo x1, x2, x3); returns false, keeping the storage order, when the centroids do
 // not form a complete lattice or the descriptor has no coordinates.
+bool tiled_order_lattice(const std::vector<int64_t> idx[3], const int64_t nax[3], int64_t E, int T,
+                         std::vector<int32_t> &order);
+
 bool tiled_order(const cmdg_desc *d, int Np, int T, std::vector<int32_t> &order) {
     const int64_t E = d->nelem;
     if (T <= 0 || d->nvgeo < 15 || E > INT32_MAX) return false;
@@ -372,7 +375,14 @@ bool tiled_order(const cmdg_desc *d, int Np, int T, std::vector<int32_t> &order)
             idx[a][e] = it - lev.begin();
         }
     }
-    if (nax[0] * nax[1] * nax[2] != E) return false;
+    return tiled_order_lattice(idx, nax, E, T, order);
+}
+
+// The tile order of elements at lattice positions idx[a][e] of an nax[0] x nax[1] x
+// nax[2] lattice (false when the positions do not fill the lattice once each).
+bool tiled_order_lattice(const std::vector<int64_t> idx[3], const int64_t nax[3], int64_t E, int T,
+                         std::vector<int32_t> &order) {
+    if (T <= 0 || E > INT32_MAX || nax[0] * nax[1] * nax[2] != E) return false;
     const int64_t ntx = (nax[0] + T - 1) / T, ntz = (nax[2] + T - 1) / T;
     std::vector<std::pair<int64_t, int32_t>> key(E);
     std::vector<char> seen(E, 0);
@@ -450,6 +460,68 @@ bool detect_affine(const cmdg_desc *d, int Nq, std::vector<double> &geo, std::ve
     return true;
 }
 
+// The per-element records of detect_affine for a brick given by its break points:
+// the values the host grid builder puts in vgeo/sgeo (xi = 2/h, J = (h1/2)(h2/2)(h3/2),
+// face Jacobians (h_b/2)(h_c/2)), the neighbours of the stacked element order, and
+// Delta from the auxiliary state (constant per element).  Returns an error text, empty
+// on success.
+std::string brick_geometry(const cmdg_desc *d, const cmdg_brick *b, int Nq, std::vector<double> &geo,
+                           std::vector<int32_t> &nbe, std::vector<double> &W, std::vector<double> &Wf,
+                           std::vector<int64_t> idx[3]) {
+    const int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+    const int64_t E = d->nelem, nx = b->ne[0], ny = b->ne[1], nz = b->ne[2];
+    if (nx <= 0 || ny <= 0 || nz <= 0 || nx * ny * nz != E) return "brick element counts do not match nelem";
+    for (int a = 0; a < 3; ++a) {
+        if (!b->x[a]) return "missing brick break points";
+        for (int64_t m = 0; m < b->ne[a]; ++m)
+            if (!(b->x[a][m + 1] > b->x[a][m])) return "brick break points must increase";
+        if (!b->periodic[a] && (b->tag[a][0] == 0 || b->tag[a][1] == 0))
+            return "a non-periodic brick face needs a non-zero boundary tag";
+    }
+    const double *w = d->omega;
+    W.resize(Np);
+    Wf.resize(Nfp);
+    for (int n = 0; n < Np; ++n) W[n] = (w[n / (Nq * Nq)] * w[(n / Nq) % Nq]) * w[n % Nq];
+    for (int fn = 0; fn < Nfp; ++fn) Wf[fn] = w[fn / Nq] * w[fn % Nq];
+    geo.assign((size_t)AFF_GEO * E, 0.0);
+    nbe.assign((size_t)6 * E, -1);
+    for (int a = 0; a < 3; ++a) idx[a].resize(E);
+    for (int64_t e = 0; e < E; ++e) {
+        const int64_t p[3] = {(e / nz) % nx, e / (nz * nx), e % nz};
+        double h[3];
+        for (int a = 0; a < 3; ++a) {
+            idx[a][e] = p[a];
+            h[a] = b->x[a][p[a] + 1] - b->x[a][p[a]];
+        }
+        double *g = &geo[(size_t)AFF_GEO * e];
+        g[0] = 2.0 / h[0];
+        g[1] = 2.0 / h[1];
+        g[2] = 2.0 / h[2];
+        g[3] = ((h[0] / 2) * (h[1] / 2)) * (h[2] / 2);
+        const double sJ[3] = {(h[1] / 2) * (h[2] / 2), (h[0] / 2) * (h[2] / 2), (h[0] / 2) * (h[1] / 2)};
+        for (int f = 0; f < 6; ++f) {
+            const int a = f / 2;
+            g[4 + f] = sJ[a];
+            int64_t q[3] = {p[0], p[1], p[2]};
+            q[a] += (f & 1) ? 1 : -1;
+            if (q[a] < 0 || q[a] >= b->ne[a]) {
+                if (!b->periodic[a]) continue;  // a wall: no neighbour
+                q[a] = (q[a] + b->ne[a]) % b->ne[a];
+            }
+            nbe[6 * e + f] = (int32_t)(q[2] + nz * (q[0] + nx * q[1]));
+        }
+        if (d->aux_Delta >= 0) {
+            const double *Dl = d->aux_init + (int64_t)Np * (d->aux_Delta + (int64_t)d->naux * e);
+            for (int n = 1; n < Np; ++n)
+                if (Dl[n] != Dl[0]) return "Delta must be constant in each element of a brick";
+            g[10] = Dl[0];
+        }
+    }
+    return "";
+}
+
+int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_ctx **out);
+
 }  // namespace
 
 extern "C" {
@@ -458,7 +530,21 @@ const char *cmdg_version(void) { return "cmdg-mi355x 0.1 (gfx950)"; }
 
 const char *cmdg_last_error(cmdg_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
+int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) { return create_impl(d, nullptr, stream, out); }
+
+int cmdg_create_brick(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_ctx **out) {
+    if (!brick) {
+        if (out) *out = nullptr;
+        return CMDG_EINVAL;
+    }
+    return create_impl(d, brick, stream, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_ctx **out) {
     if (!out) return CMDG_EINVAL;
     *out = nullptr;
     if (!d) return CMDG_EINVAL;
@@ -466,9 +552,10 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     if (!c) return CMDG_ENOMEM;
     *out = c;
     if (d->N != 3 && d->N != 4) return fail(c, CMDG_EUNSUPPORTED, "only N = 3 and N = 4 are compiled");
-    if (d->nelem <= 0 || d->nvgeo < 11 || d->naux < 0) return fail(c, CMDG_EINVAL, "bad sizes");
-    if (!d->vgeo || !d->sgeo || !d->vmap_minus || !d->vmap_plus || !d->elemtobndy || !d->D)
+    if (d->nelem <= 0 || (!brick && d->nvgeo < 11) || d->naux < 0) return fail(c, CMDG_EINVAL, "bad sizes");
+    if (!d->D || !(brick || (d->vgeo && d->sgeo && d->vmap_minus && d->vmap_plus && d->elemtobndy)))
         return fail(c, CMDG_EINVAL, "missing geometry or connectivity");
+    if (brick && !d->omega) return fail(c, CMDG_EINVAL, "a brick needs the quadrature weights omega");
     if (d->precision != CMDG_FP64 && d->precision != CMDG_MIXED) return fail(c, CMDG_EINVAL, "unknown precision");
     if (d->turbulence == CMDG_LEARNED_CLOSURE && d->precision != CMDG_MIXED)
         return fail(c, CMDG_EUNSUPPORTED, "the learned closure runs in the mixed-precision path");
@@ -526,7 +613,7 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     const int64_t E = c->nelem;
     // connectivity: minus side must be the element's own face nodes (Grids.jl:615)
     const int Nq = c->Nq, NN = c->N;
-    for (int64_t e = 0; e < E; ++e)
+    for (int64_t e = 0; e < (brick ? 0 : E); ++e)
         for (int f = 0; f < 6; ++f)
             for (int fn = 0; fn < Nfp; ++fn) {
                 int a = fn % Nq, b = fn / Nq, vid;
@@ -557,8 +644,13 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
         std::vector<double> geo, W, Wf;
         std::vector<int32_t> nbe;
         const char *env = std::getenv("CMDG_GENERAL_GEOMETRY");
-        const bool force_general = env && env[0] == '1';
-        if (!force_general && detect_affine(d, Nq, geo, nbe, W, Wf)) {
+        const bool force_general = env && env[0] == '1' && !brick;  // a brick has no per-node arrays
+        std::vector<int64_t> lat[3];
+        if (brick) {
+            const std::string why = brick_geometry(d, brick, Nq, geo, nbe, W, Wf, lat);
+            if (!why.empty()) return fail(c, CMDG_EINVAL, why);
+        }
+        if (brick || (!force_general && detect_affine(d, Nq, geo, nbe, W, Wf))) {
             c->affine = true;
             if ((rc = upload(c, &c->d_geo, geo.data(), geo.size()))) return rc;
             if ((rc = upload(c, &c->d_nbe, nbe.data(), nbe.size()))) return rc;
@@ -631,7 +723,8 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
             const char *tile_env = std::getenv("CMDG_ELEMENT_TILE");
             const int T = tile_env ? std::atoi(tile_env) : 8;
             std::vector<int32_t> order;
-            if (tiled_order(d, Np, T, order)) {
+            const int64_t nax[3] = {brick ? brick->ne[0] : 0, brick ? brick->ne[1] : 0, brick ? brick->ne[2] : 0};
+            if (brick ? tiled_order_lattice(lat, nax, E, T, order) : tiled_order(d, Np, T, order)) {
                 if ((rc = upload(c, &c->d_order, order.data(), order.size()))) return rc;
                 const char *penv = std::getenv("CMDG_TILED_PASSES");
                 const std::string passes = penv ? penv : "tendency";
@@ -700,6 +793,10 @@ int cmdg_create(const cmdg_desc *d, void *stream, cmdg_ctx **out) {
     }
     return CMDG_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int cmdg_destroy(cmdg_ctx *c) {
     if (!c) return CMDG_OK;
@@ -1020,6 +1117,23 @@ int cmdg_weightedsum(cmdg_ctx *c, const double *Q, int32_t state, double *out) {
                        c->affine ? c->d_geo : nullptr, c->d_W, state, c->nelem, c->d_part);
     hipLaunchKernelGGL(k_dd_reduce, dim3(1), dim3(256), 0, c->stream, c->d_part, (int)nb, c->d_scalar);
     return diag_result(c, out);
+}
+
+int cmdg_not_finite_fields(cmdg_ctx *c, const double *Q, int32_t *mask) {
+    if (!c) return CMDG_EINVAL;
+    if (!Q || !mask) return fail(c, CMDG_EINVAL, "null pointer");
+    const int64_t total = (int64_t)c->Np * 5 * c->nelem;
+    int64_t nb = (total + 255) / 256;
+    if (nb > 1024) nb = 1024;
+    int rc = diag_scratch(c, nb);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_not_finite, dim3((unsigned)nb), dim3(256), 0, c->stream, Q, c->Np, 5, total, c->d_part);
+    hipLaunchKernelGGL(k_or_reduce, dim3(1), dim3(256), 0, c->stream, c->d_part, (int)nb, c->d_scalar);
+    double m = 0.0;
+    rc = diag_result(c, &m);
+    if (rc) return rc;
+    *mask = (int32_t)m;
+    return CMDG_OK;
 }
 
 int cmdg_geometry_kind(cmdg_ctx *c, int32_t *kind) {

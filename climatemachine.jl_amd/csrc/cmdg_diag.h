@@ -153,6 +153,41 @@ __global__ void __launch_bounds__(256) k_weightedsum(const double *__restrict__ 
     if (threadIdx.x == 0) partial[blockIdx.x] = make_double2(red[0].hi, red[0].lo);
 }
 
+// Which of the nstate fields of Q (Np, nstate, nelem) hold a non-finite value:
+// bit s of the block's mask is set when some Q[:, s, :] in its share is NaN or Inf
+// (the map-reduce of show_not_finite_fields, src/Arrays/MPIStateArrays.jl:884-903).
+// Each block writes its mask (as a double, exact) to partial[blockIdx].x; k_or_reduce
+// combines them.  Plain vector stores only.
+__global__ void __launch_bounds__(256) k_not_finite(const double *__restrict__ Q, int Np, int nstate, int64_t total,
+                                                    double2 *__restrict__ partial) {
+    __shared__ unsigned red[256];
+    unsigned mask = 0u;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (int64_t)gridDim.x * 256) {
+        const double v = Q[i];
+        if (!isfinite(v)) mask |= 1u << (int)((i / Np) % nstate);
+    }
+    red[threadIdx.x] = mask;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] |= red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = make_double2((double)red[0], 0.0);
+}
+
+__global__ void __launch_bounds__(256) k_or_reduce(const double2 *__restrict__ partial, int n, double *__restrict__ out) {
+    __shared__ unsigned red[256];
+    unsigned mask = 0u;
+    for (int i = threadIdx.x; i < n; i += 256) mask |= (unsigned)partial[i].x;
+    red[threadIdx.x] = mask;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] |= red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (double)red[0];
+}
+
 __global__ void __launch_bounds__(256) k_dd_reduce(const double2 *__restrict__ partial, int n, double *__restrict__ out) {
     __shared__ DD red[256];
     DD acc{0.0, 0.0};

@@ -9,7 +9,13 @@
  *                                      xi3x3, M, MI, MH, x1, x2, x3, JcV[, x_xi]
  *   sgeo        (5, Nfp, 6, nelem)     n1, n2, n3, sM, vMI
  *   vmap-/vmap+ (Nfp, 6, nelem) int64  1-based linear volume dof indices
- *   elemtobndy  (6, nelem) int64       0 = connected face, else boundary tag
+ *   elemtobndy  (6, nelem) int64       0 = connected face, else boundary tag; every
+ *                                      non-zero tag is the one boundary condition the
+ *                                      library implements, the reference's default
+ *                                      AtmosBC(Impenetrable(FreeSlip()), Insulating())
+ *                                      (src/Atmos/Model/boundaryconditions.jl:43-110):
+ *                                      a caller whose boundary-condition tuple holds
+ *                                      other conditions cannot use this library
  *   D           (Nq, Nq)               D[j + Nq*k] = l_k'(xi_j)
  * Reference layout sources:
  *   src/Numerics/Mesh/Grids.jl:76-130 (vgeo/sgeo field ids),
@@ -104,12 +110,37 @@ typedef struct cmdg_ctx cmdg_ctx;
  * the auxiliary state (DGModel.state_auxiliary) and allocates scratch (gradient
  * flux, ping-pong state).  `stream` is a hipStream_t (NULL = default stream). */
 int cmdg_create(const cmdg_desc *desc, void *stream, cmdg_ctx **out);
+
+/* An axis-aligned brick of ne[0] x ne[1] x ne[2] elements with break points x[a]
+ * (ne[a] + 1 increasing values along axis a), stored with the vertical index fastest:
+ * element e = k + ne[2] (i + ne[0] j) at lattice position (i, j, k), the order of the
+ * reference's StackedBrickTopology (src/Numerics/Mesh/Topologies.jl:632-).  periodic[a]
+ * joins the two ends of axis a; otherwise tag[a][0] / tag[a][1] is the boundary tag of
+ * its lower / upper faces (non-zero: a wall, as any non-zero elemtobndy tag). */
+typedef struct cmdg_brick {
+    int64_t ne[3];
+    const double *x[3];
+    int32_t periodic[3];
+    int64_t tag[3][2];
+} cmdg_brick;
+
+/* cmdg_create for a brick described by `brick` instead of per-node arrays (no
+ * reference counterpart: a drop-in caller passes the grid's vgeo/sgeo/vmaps to
+ * cmdg_create).  desc's vgeo, sgeo, vmap_minus, vmap_plus, elemtobndy and nvgeo are
+ * ignored and may be null; D, omega and aux_init are read as by cmdg_create.  The
+ * context is the one cmdg_create builds from the same brick's arrays (metric terms
+ * 2/h, J = h1 h2 h3 / 8, the same element records, classes and traversal order), so
+ * results are bitwise the same; it saves building and passing the 25 + 2 x 6 x 5 per-node
+ * words of geometry and connectivity of a large mesh. */
+int cmdg_create_brick(const cmdg_desc *desc, const cmdg_brick *brick, void *stream, cmdg_ctx **out);
 int cmdg_destroy(cmdg_ctx *ctx);
 
 /* tendency = alpha * dQ/dt(Q, t) + beta * tendency
  * replaces (dg::DGModel)(tendency, Q, _, t, alpha, beta),
  * src/Numerics/DGMethods/DGModel.jl:85-427; the auxiliary theta_v / air_T
- * are refreshed from Q as in DGModel.jl:110-116. */
+ * are refreshed from Q as in DGModel.jl:110-116.  t is accepted for the
+ * reference's signature; no flux, source or boundary condition the library
+ * implements depends on time, so the tendency does not either. */
 int cmdg_rhs(cmdg_ctx *ctx, double *tendency, const double *Q, double t,
              double alpha, double beta);
 
@@ -238,6 +269,13 @@ int cmdg_courant(cmdg_ctx *ctx, const double *Q, double dt, int32_t kind, int32_
  * M) accumulated in double-double: weightedsum(Q, state),
  * src/Arrays/MPIStateArrays.jl:655-674.  Synchronises; *sum is a host double. */
 int cmdg_weightedsum(cmdg_ctx *ctx, const double *Q, int32_t state, double *sum);
+
+/* Which state fields of Q (device, (Np, 5, nelem)) hold a NaN or Inf: bit s of *mask
+ * is set when some Q[:, s, :] is not finite (0: all finite).  The map-reduce of
+ * show_not_finite_fields (src/Arrays/MPIStateArrays.jl:884-903), done on the device;
+ * a solver loop can call it every few steps to stop on a crash (the rhs and step
+ * entry points themselves do not check).  Synchronises; *mask is a host int. */
+int cmdg_not_finite_fields(cmdg_ctx *ctx, const double *Q, int32_t *mask);
 
 /* Geometry representation the context chose (diagnostic; no reference counterpart),
  * a bit set: bit 0 = axis-aligned affine bricks (per-element metrics, detected bitwise

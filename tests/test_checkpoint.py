@@ -70,3 +70,14 @@ def test_tensor_input_uses_the_device_view(tmp_path):
     ck.write_checkpoint(tQ, tA, 1.0, str(tmp_path), "t", 1)
     q2, a2, _ = ck.read_checkpoint(str(tmp_path), "t", 1)
     assert np.array_equal(q2, Q) and np.array_equal(a2, A)
+
+
+def test_superblock_version_2_after_the_user_block(tmp_path):
+    """JLD2 writes HDF5 superblock version 2 (file format objects of HDF5 1.8): the
+    signature sits right after the 512-byte user block, followed by the version byte."""
+    Q = np.asfortranarray(np.ones((125, 5, 2)))
+    p = ck.write_checkpoint(Q, Q, 0.0, str(tmp_path), "sb", 0)
+    with open(p, "rb") as f:
+        raw = f.read(ck.USERBLOCK + 9)
+    assert raw[ck.USERBLOCK:ck.USERBLOCK + 8] == b"\x89HDF\r\n\x1a\n"
+    assert raw[ck.USERBLOCK + 8] == 2
