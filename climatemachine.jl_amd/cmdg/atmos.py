@@ -26,7 +26,7 @@ from typing import Optional
 import numpy as np
 
 from . import abi
-from .grid import BrickGrid, MI_ID, M_ID, X1_ID, X2_ID, X3_ID
+from .grid import BrickGrid, MI_ID, M_ID, X1_ID, X2_ID, X3_ID, over_elements
 
 
 # --------------------------------------------------------------------- parameters
@@ -383,21 +383,25 @@ def taylor_green(prm: Parameters, grid: BrickGrid, mach: float = 0.1, rho0: floa
     """Taylor-Green vortex on a periodic (2 pi L)^3 box: u = U sin x cos y cos z,
     v = -U cos x sin y cos z, w = 0, p = p0 + rho0 U^2/16 (cos 2x + cos 2y)(cos 2z + 2),
     with U = Mach * c(T0) and seeded phase shifts (experiments/AtmosLES/taylor_green.jl:50-83)."""
-    x, y, z = (c / L for c in grid.coords)
-    x, y, z = x + phase[0], y + phase[1], z + phase[2]
     c0 = np.sqrt(prm.cp_d / prm.cv_d * prm.R_d * T0)
     U = mach * c0
     p0 = rho0 * prm.R_d * T0
-    u = U * np.sin(x) * np.cos(y) * np.cos(z)
-    v = -U * np.cos(x) * np.sin(y) * np.cos(z)
-    w = 0 * u
-    p = p0 + rho0 * U ** 2 / 16 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2)
-    rho = rho0 * p / p0          # isothermal start
-    T = p / (prm.R_d * rho)
+    X = grid.coords
     Q = np.empty((grid.Np, 5, grid.nelem), order="F")
-    Q[:, 0, :] = rho
-    Q[:, 1, :] = rho * u
-    Q[:, 2, :] = rho * v
-    Q[:, 3, :] = rho * w
-    Q[:, 4, :] = rho * total_energy(prm, (u * u + v * v + w * w) / 2, 0.0, T)
+
+    def fill(e0, e1):
+        x, y, z = (c[:, e0:e1] / L for c in X)
+        x, y, z = x + phase[0], y + phase[1], z + phase[2]
+        u = U * np.sin(x) * np.cos(y) * np.cos(z)
+        v = -U * np.cos(x) * np.sin(y) * np.cos(z)
+        w = 0 * u
+        p = p0 + rho0 * U ** 2 / 16 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2)
+        rho = rho0 * p / p0          # isothermal start
+        T = p / (prm.R_d * rho)
+        Q[:, 0, e0:e1] = rho
+        Q[:, 1, e0:e1] = rho * u
+        Q[:, 2, e0:e1] = rho * v
+        Q[:, 3, e0:e1] = rho * w
+        Q[:, 4, e0:e1] = rho * total_energy(prm, (u * u + v * v + w * w) / 2, 0.0, T)
+    over_elements(grid.nelem, fill)
     return Q

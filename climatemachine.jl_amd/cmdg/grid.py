@@ -22,7 +22,24 @@ construction itself (``curvilinear_geometry``).
 """
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
+
+
+def over_elements(nelem: int, fn, min_chunk: int = 4096) -> None:
+    """Call fn(e0, e1) on consecutive element ranges covering [0, nelem), on a thread
+    pool when the mesh is large.  numpy's element-wise operations release the GIL, so
+    host arrays of a large mesh fill in parallel, with the same values a single call
+    fn(0, nelem) gives (every range evaluates the same operations on its own nodes)."""
+    workers = min(16, os.cpu_count() or 1)
+    if workers <= 1 or nelem < 2 * min_chunk:
+        fn(0, nelem)
+        return
+    n = max(min_chunk, -(-nelem // (4 * workers)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        list(ex.map(lambda e0: fn(e0, min(nelem, e0 + n)), range(0, nelem, n)))
 
 # vgeo field ids, 0-based (Grids.jl:76-92; GeometricFactors.jl:18-67)
 XI1X1, XI2X1, XI3X1, XI1X2, XI2X2, XI3X2, XI1X3, XI2X3, XI3X3 = range(9)
@@ -367,8 +384,14 @@ class BrickGrid:
             return (self.vgeo[:, X1_ID, :], self.vgeo[:, X2_ID, :], self.vgeo[:, X3_ID, :])
         if self._coords is None:
             r = self.xi
-            self._coords = tuple(np.asfortranarray(self._x0[d][None, :] + ((r[self.node_ijk[d]] + 1) / 2)[:, None]
-                                                   * self._h[d][None, :]) for d in range(3))
+            xs = tuple(np.empty((self.Np, self.nelem), order="F") for _ in range(3))
+            rr = [((r[self.node_ijk[d]] + 1) / 2)[:, None] for d in range(3)]
+
+            def fill(e0, e1):
+                for d in range(3):
+                    xs[d][:, e0:e1] = self._x0[d][None, e0:e1] + rr[d] * self._h[d][None, e0:e1]
+            over_elements(self.nelem, fill)
+            self._coords = xs
         return self._coords
 
     @property
