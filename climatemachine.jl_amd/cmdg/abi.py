@@ -14,7 +14,8 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 ROOT_DIR = os.path.dirname(PKG_DIR)          # climatemachine.jl_amd/
 LIB_PATH = os.path.join(ROOT_DIR, "lib", "libcmdg_hip.so")
 
-CMDG_OK, CMDG_EINVAL, CMDG_EHIP, CMDG_ENOMEM, CMDG_EUNSUPPORTED = range(5)
+CMDG_OK, CMDG_EINVAL, CMDG_EHIP, CMDG_ENOMEM, CMDG_EUNSUPPORTED, CMDG_ERANGE = range(6)
+MLP_STATIC_MAX_BATCH = 65536   # CMDG_MLP_STATIC_MAX_BATCH
 NO_ORIENTATION, FLAT_ORIENTATION = 0, 1
 NO_REFERENCE_STATE, HYDROSTATIC_STATE = 0, 1
 CONSTANT_DYNAMIC_VISCOSITY, CONSTANT_KINEMATIC_VISCOSITY, SMAGORINSKY_LILLY, LEARNED_CLOSURE = range(4)
@@ -71,7 +72,7 @@ EXPORTED = (
     "cmdg_create", "cmdg_create_brick", "cmdg_destroy", "cmdg_rhs", "cmdg_lsrk54_step", "cmdg_lsrk54_steps",
     "cmdg_lsrk_stage", "cmdg_lsrk_update", "cmdg_aux", "cmdg_gradient_flux",
     "cmdg_gradient_flux_f32", "cmdg_mlp_set_weights", "cmdg_mlp_forward", "cmdg_velocity_gradient",
-    "cmdg_box_filter", "cmdg_closure_samples", "cmdg_mlp_loss_grad", "cmdg_mlp_adam", "cmdg_interpolate_equispaced",
+    "cmdg_box_filter", "cmdg_closure_samples", "cmdg_mlp_loss_grad", "cmdg_mlp_loss_grad_static", "cmdg_mlp_adam", "cmdg_interpolate_equispaced",
     "cmdg_courant", "cmdg_weightedsum", "cmdg_not_finite_fields", "cmdg_geometry_kind", "cmdg_set_kernel_timing", "cmdg_kernel_timing", "cmdg_sync",
     "cmdg_last_error", "cmdg_version",
 )
@@ -112,6 +113,7 @@ def load_library(path: str = None):
     lib.cmdg_closure_samples.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_double, C.c_double, C.c_double,
                                          C.c_int32, vp, C.c_int64, C.c_int64, vp]
     lib.cmdg_mlp_loss_grad.argtypes = [vp, vp, vp, vp, C.c_int64, vp, C.c_int64, C.c_double, vp, vp, C.c_int64, vp]
+    lib.cmdg_mlp_loss_grad_static.argtypes = [vp, vp, vp, vp, C.c_int64, vp, C.c_int64, C.c_double, vp, vp]
     lib.cmdg_mlp_adam.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int64,
                                   vp]
     lib.cmdg_interpolate_equispaced.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, C.c_int64, C.c_int64,

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <string_view>
@@ -1014,6 +1015,64 @@ int cmdg_mlp_loss_grad(const float *params, const float *shift, const float *sca
     if (!idx && batch > ld) return CMDG_EINVAL;
     const int64_t nb = (batch + TRAIN_TB - 1) / TRAIN_TB;
     if (nb * GRAD_W > work_floats) return CMDG_EINVAL;
+    hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)nb), dim3(TRAIN_TB), 0, (hipStream_t)stream, params, shift, scale,
+                       samples, ld, idx, batch, (float)inv_norm, work);
+    if (hipGetLastError() != hipSuccess) return CMDG_EHIP;
+    hipLaunchKernelGGL(k_grad_reduce, dim3((GRAD_W + 255) / 256), dim3(256), 0, (hipStream_t)stream, work, nb, grad);
+    return hipGetLastError() == hipSuccess ? CMDG_OK : CMDG_EHIP;
+}
+
+}  // extern "C"
+
+// Partial gradients of cmdg_mlp_loss_grad_static: one GRAD_W row per workgroup of
+// TRAIN_TB samples, for at most MLP_STATIC_BLOCKS workgroups.  Each GPU holds its own
+// copy; k_mlp_grad writes every entry of the rows k_grad_reduce reads, so nothing in
+// it has to start at zero.
+constexpr int64_t MLP_STATIC_BLOCKS = CMDG_MLP_STATIC_MAX_BATCH / TRAIN_TB;
+static_assert(MLP_STATIC_BLOCKS * TRAIN_TB == CMDG_MLP_STATIC_MAX_BATCH, "whole workgroups");
+static __device__ __align__(256) float scratch[MLP_STATIC_BLOCKS * GRAD_W];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on the device of `stream` (the current device for
+// the null stream), looked up once per device with that device current.
+int static_scratch(hipStream_t stream, float **out) {
+    int dev = -1, cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return CMDG_EHIP;
+    dev = cur;
+    if (stream && hipStreamGetDevice(stream, &dev) != hipSuccess) return CMDG_EHIP;
+    if (dev < 0 || dev >= kMaxDevices) return CMDG_EUNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        if (dev != cur && hipSetDevice(dev) != hipSuccess) return CMDG_EHIP;
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (dev != cur && hipSetDevice(cur) != hipSuccess) return CMDG_EHIP;
+        if (e != hipSuccess || !p) return CMDG_EHIP;
+        g_scratch_dev[dev] = static_cast<float *>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return CMDG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cmdg_mlp_loss_grad_static(const float *params, const float *shift, const float *scale, const float *samples,
+                              int64_t ld, const int64_t *idx, int64_t batch, double inv_norm, float *grad,
+                              void *stream) {
+    if (batch <= 0 || !params || !shift || !scale || !samples || !grad) return CMDG_EINVAL;
+    if (!idx && batch > ld) return CMDG_EINVAL;
+    if (batch > CMDG_MLP_STATIC_MAX_BATCH) return CMDG_ERANGE;
+    float *work = nullptr;
+    const int rc = static_scratch((hipStream_t)stream, &work);
+    if (rc) return rc;
+    const int64_t nb = (batch + TRAIN_TB - 1) / TRAIN_TB;
     hipLaunchKernelGGL(k_mlp_grad, dim3((unsigned)nb), dim3(TRAIN_TB), 0, (hipStream_t)stream, params, shift, scale,
                        samples, ld, idx, batch, (float)inv_norm, work);
     if (hipGetLastError() != hipSuccess) return CMDG_EHIP;

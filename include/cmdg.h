@@ -44,6 +44,7 @@ extern "C" {
 #define CMDG_EHIP 2
 #define CMDG_ENOMEM 3
 #define CMDG_EUNSUPPORTED 4
+#define CMDG_ERANGE 5 /* a size beyond an entry point's fixed maximum */
 
 /* orientation: src/Common/Orientations/Orientations.jl:138-146, 246-257 */
 #define CMDG_NO_ORIENTATION 0
@@ -239,6 +240,21 @@ int cmdg_mlp_loss_grad(const float *params, const float *shift, const float *sca
                        const float *samples, int64_t ld, const int64_t *idx, int64_t batch,
                        double inv_norm, float *grad, float *work, int64_t work_floats,
                        void *stream);
+
+/* cmdg_mlp_loss_grad without the work argument: the same kernels and the same
+ * result, bit for bit, with the per-workgroup partial gradients kept in a fixed
+ * device array of the library (5.6 MiB per GPU, held until the process ends) instead
+ * of caller-allocated scratch.  No reference counterpart: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc); it calls no hipMalloc, hipFree or allocating
+ * library routine.  batch is at most CMDG_MLP_STATIC_MAX_BATCH (CMDG_ERANGE above
+ * it).  Every call on one GPU shares that GPU's array: calls that overlap in time on
+ * one GPU (other streams or threads) race and must be ordered by the caller.
+ * Enqueues on `stream` (a hipStream_t of the device the array is taken from). */
+#define CMDG_MLP_STATIC_MAX_BATCH 65536
+int cmdg_mlp_loss_grad_static(const float *params, const float *shift, const float *scale,
+                              const float *samples, int64_t ld, const int64_t *idx, int64_t batch,
+                              double inv_norm, float *grad, void *stream);
 
 /* One Adam step (bias-corrected, step >= 1) on n parameters. */
 int cmdg_mlp_adam(float *params, const float *grad, float *m, float *v, int64_t n, float lr,

@@ -100,3 +100,20 @@ def test_solve_batches_steps_like_the_step_loop(kw):
     assert [c[0] for c in got.calls] == [c[0] for c in ref.calls]
     for a, b in zip(got.calls, ref.calls):
         assert a[1] == pytest.approx(b[1], abs=1e-15) and a[2] == b[2]
+
+
+def test_static_loss_gradient_rejects_oversized_batches_without_gpu_work():
+    """The size check of the allocation-free loss gradient comes before any device call."""
+    lib = abi.load_library()
+    p = C.c_void_p(256)
+    n = abi.MLP_STATIC_MAX_BATCH + 1
+    assert lib.cmdg_mlp_loss_grad_static(p, p, p, p, n, None, n, 1.0, p, None) == abi.CMDG_ERANGE
+    assert lib.cmdg_mlp_loss_grad_static(p, p, p, p, 10, None, 0, 1.0, p, None) == abi.CMDG_EINVAL
+
+
+def test_header_constants_match_the_ctypes_mirror():
+    text = open(HEADER).read()
+    consts = dict(re.findall(r"^#define (CMDG_\w+) (\d+)", text, re.M))
+    assert int(consts["CMDG_ERANGE"]) == abi.CMDG_ERANGE
+    assert int(consts["CMDG_MLP_STATIC_MAX_BATCH"]) == abi.MLP_STATIC_MAX_BATCH
+    assert int(consts["CMDG_MLP_NPARAM"]) == abi.MLP_NPARAM
