@@ -53,12 +53,12 @@ WEIGHTS = os.path.join(ROOT, "climatemachine.jl_amd", "cmdg", "data", "tgv_closu
 TRAIN_META = os.path.join(ROOT, "climatemachine.jl_amd", "cmdg", "data", "tgv_closure_train.json")
 COUNTS = os.path.join(ROOT, "climatemachine.jl_amd", "cmdg", "data", "algorithmic_counts.json")
 # counters of the two passes (rocprofv3 collects at most 8 SQ and 4 TCC counters a run)
+# (the SQ_INSTS_VALU_FLOPS_* counters are not used: they count ADD + MUL + TRANS + 2 FMA per
+# wave instruction, 1/64 of the lanes' flops, which the instruction counters already give)
 PASSES = (("FETCH_SIZE", "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_FMA_F32",
-           "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_MFMA_MOPS_BF16", "SQ_INSTS_VALU_FLOPS_FP32",
-           "SQ_INSTS_VALU_FLOPS_FP64", "SQ_WAVES"),
+           "SQ_INSTS_VALU_TRANS_F32", "SQ_INSTS_VALU_MFMA_MOPS_BF16", "SQ_WAVES"),
           ("WRITE_SIZE", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64", "SQ_INSTS_VALU_FMA_F64",
-           "SQ_INSTS_VALU_TRANS_F64", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "SQ_INSTS_VALU",
-           "SQ_INSTS_VALU_FLOPS_FP32_TRANS"))
+           "SQ_INSTS_VALU_TRANS_F64", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "SQ_INSTS_VALU"))
 
 
 def build_les(ne=25):
@@ -297,12 +297,7 @@ def flop_rates(cnt, avg_ms):
     f32, f64 = per("F32"), per("F64")
     mfma = 512 * cnt.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0)
     s = avg_ms * 1e-3
-    direct = {}
-    for k, name in (("FP32", "SQ_INSTS_VALU_FLOPS_FP32"), ("FP64", "SQ_INSTS_VALU_FLOPS_FP64"),
-                    ("FP32_TRANS", "SQ_INSTS_VALU_FLOPS_FP32_TRANS")):
-        if name in cnt:
-            direct[k.lower() + "_tflops_flops_counter"] = round(cnt[name] / s / 1e12, 3)
-    return {**direct, "fp32_tflops": round(f32 / s / 1e12, 3), "fp64_tflops": round(f64 / s / 1e12, 3),
+    return {"fp32_tflops": round(f32 / s / 1e12, 3), "fp64_tflops": round(f64 / s / 1e12, 3),
             "bf16_mfma_tflops": round(mfma / s / 1e12, 3),
             "fp32_frac": round(f32 / s / 1e12 / PEAK_TFLOPS["fp32"], 4),
             "fp64_frac": round(f64 / s / 1e12 / PEAK_TFLOPS["fp64"], 4),
