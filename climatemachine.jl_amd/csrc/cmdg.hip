@@ -164,27 +164,35 @@ int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)xcd_grid(c->affine ? ES.count : c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_GRADIENT);
+    // N^2 (Smagorinsky-Lilly) needs theta_v only under a gravitational potential
+    const bool buoy = c->P.a_gPhi >= 0;
+#define CMDG_GRAD_MIX(SMAG, LEARN, BUOY)                                                                       \
+    hipLaunchKernelGGL((k_gradient_mix<Nq, SMAG, LEARN, BUOY>), grid, block, 0, c->stream, c->P, c->MP, c->AA, ES, \
+                       Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp)
+#define CMDG_GRAD_AFF(SMAG, BUOY)                                                                               \
+    hipLaunchKernelGGL((k_gradient_aff<Nq, SMAG, BUOY>), grid, block, 0, c->stream, c->P, c->AA, ES, Q, c->d_aux, \
+                       c->d_gf, c->d_f2tr, write_aux)
     if (c->mixed) {
-        if (c->learned)
-            hipLaunchKernelGGL((k_gradient_mix<Nq, true, true>), grid, block, 0, c->stream, c->P, c->MP, c->AA, ES,
-                               Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
-        else if (c->smag)
-            hipLaunchKernelGGL((k_gradient_mix<Nq, true, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA, ES,
-                               Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
-        else
-            hipLaunchKernelGGL((k_gradient_mix<Nq, false, false>), grid, block, 0, c->stream, c->P, c->MP, c->AA,
-                               ES, Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp);
+        if (c->learned) {
+            if (buoy) CMDG_GRAD_MIX(true, true, true);
+            else CMDG_GRAD_MIX(true, true, false);
+        } else if (c->smag) {
+            if (buoy) CMDG_GRAD_MIX(true, false, true);
+            else CMDG_GRAD_MIX(true, false, false);
+        } else
+            CMDG_GRAD_MIX(false, false, false);
     } else if (c->affine) {
-        if (c->smag)
-            hipLaunchKernelGGL((k_gradient_aff<Nq, true>), grid, block, 0, c->stream, c->P, c->AA, ES, Q,
-                               c->d_aux, c->d_gf, c->d_f2tr, write_aux);
-        else
-            hipLaunchKernelGGL((k_gradient_aff<Nq, false>), grid, block, 0, c->stream, c->P, c->AA, ES, Q,
-                               c->d_aux, c->d_gf, c->d_f2tr, write_aux);
+        if (c->smag) {
+            if (buoy) CMDG_GRAD_AFF(true, true);
+            else CMDG_GRAD_AFF(true, false);
+        } else
+            CMDG_GRAD_AFF(false, false);
     } else if (c->smag)
         hipLaunchKernelGGL((k_gradient<Nq, true>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_aux, c->d_gf, Qcopy, write_aux);
     else
         hipLaunchKernelGGL((k_gradient<Nq, false>), grid, block, 0, c->stream, c->P, c->A, Q, c->d_aux, c->d_gf, Qcopy, write_aux);
+#undef CMDG_GRAD_MIX
+#undef CMDG_GRAD_AFF
     return check_hip(c, hipGetLastError(), "gradient launch");
 }
 

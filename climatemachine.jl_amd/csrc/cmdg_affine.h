@@ -98,7 +98,7 @@ __device__ __forceinline__ double own_aux(const KParams &P, const double *__rest
 // (DGModel_kernels.jl:1471-1647, CentralNumericalFluxGradient NumericalFluxes.jl:67-123,
 // walls: G* = n G(q_wall), bc_momentum.jl:24-43).  G- and theta_v- come from the
 // element's own gradient arguments in LDS, q+ from the neighbour's volume state.
-template <int Nq, bool SMAG>
+template <int Nq, bool SMAG, bool BUOY>
 __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArrays &A, int64_t e, int f, int fn,
                                                   int32_t ne, const double *__restrict__ Q,
                                                   const double (*sG)[Nq * Nq * Nq], double *dgf) {
@@ -127,7 +127,7 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
         for (int s = 0; s < 5; ++s) qp[s] = Q[qb + Np * s];
         const double Php = A.aux_cont ? own_aux<Nq>(P, A.sa, ce, vn, P.a_Phi, -0.0) : A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
         Thermo tp = thermo(P, qp, Php);
-        grad_arg<SMAG>(P, qp, tp, SMAG ? theta_v(P, tp) : 0.0, Gp);
+        grad_arg<SMAG>(P, qp, tp, SMAG ? (BUOY ? theta_v(P, tp) : 1.0) : 0.0, Gp);
 #pragma unroll
         for (int s = 0; s < 5; ++s) Gs[s] = nd * (Gp[s] + G[s]) * 0.5;
     } else {
@@ -138,7 +138,7 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
         const double nv[3] = {d == 0 ? nd : 0.0, d == 1 ? nd : 0.0, d == 2 ? nd : 0.0};
         wall_state(q, nv, 1.0, qb);
         Thermo tb = thermo(P, qb, Phi);
-        grad_arg<SMAG>(P, qb, tb, SMAG ? theta_v(P, tb) : 0.0, Gp);
+        grad_arg<SMAG>(P, qb, tb, SMAG ? (BUOY ? theta_v(P, tb) : 1.0) : 0.0, Gp);
 #pragma unroll
         for (int s = 0; s < 5; ++s) Gs[s] = nd * Gp[s];
     }
@@ -151,7 +151,9 @@ __device__ __forceinline__ void face_gradient_aff(const KParams &P, const AffArr
     for (int s = 0; s < NGF; ++s) dgf[s] = gfs[s] - gfm[s];
 }
 
-template <int Nq, bool SMAG>
+// BUOY: as for k_gradient_mix, theta_v and its gradient are evaluated for N^2 only when
+// the context has a gravitational potential.
+template <int Nq, bool SMAG, bool BUOY>
 __global__ void __launch_bounds__(threads_for<Nq>())
 k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q, double *__restrict__ aux,
                double *__restrict__ gfo, double *__restrict__ f2tr, int write_aux) {
@@ -173,13 +175,17 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
     const double *ge = A.geo + (int64_t)AFF_GEO * e;
     double q[5], thv = 0.0, G[5];
     Aux a;
+    constexpr bool buoy = SMAG && BUOY;
     if (act) {
         const double *qb = Q + (int64_t)Np * 5 * e + n;
 #pragma unroll
         for (int s = 0; s < 5; ++s) q[s] = qb[Np * s];
         a = load_aux(P, A.sa, Np, n, static_class(A, e));
         Thermo t = thermo(P, q, a.Phi);
-        thv = theta_v(P, t);
+        // theta_v enters N^2 = grad theta_v . grad Phi / theta_v only, zero without a
+        // gravitational potential (grad Phi = 0): then it is evaluated only for the aux
+        // refresh, and 1 stands in the gradient argument (the face terms divide by it)
+        thv = (buoy || (write_aux && P.a_theta_v >= 0)) ? theta_v(P, t) : 1.0;
         if (write_aux) {
             double *ab = aux + (int64_t)Np * P.naux * e + n;
             if (P.a_theta_v >= 0) ab[(int64_t)Np * P.a_theta_v] = thv;
@@ -193,7 +199,7 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
     // lifted central jumps of all face nodes, spread over every lane
     for (int t = n; t < 6 * Nfp; t += NT) {
         double dgf[NGF];
-        face_gradient_aff<Nq, SMAG>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, sG, dgf);
+        face_gradient_aff<Nq, SMAG, BUOY>(P, A, e, t / Nfp, t % Nfp, nb[t / Nfp], Q, sG, dgf);
 #pragma unroll
         for (int s = 0; s < NGF; ++s) sJ[t][s] = dgf[s];
     }
@@ -207,7 +213,7 @@ k_gradient_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
             double g1 = 0.0, g2 = 0.0, g3 = -0.0;
-            if (s < NG) {
+            if (s < NG && (s < 4 || buoy)) {
 #pragma unroll
                 for (int m = 0; m < Nq; ++m) {
                     g1 += sD[i + Nq * m] * sG[s][m + Nq * (j + Nq * k)];

@@ -127,26 +127,32 @@ __device__ __forceinline__ ThermoD thermo_d(const KParams &P, const double q[5],
 
 // gradient argument (u, h_tot, theta_v) in FP32 from the FP64 state and thermodynamics;
 // theta_v = T (p / MSLP)^-kappa as T 2^(-kappa log2(p / MSLP)) on the hardware log2 and
-// exp2 (a few FP32 ulp: below the FP32 rounding of theta_v that its gradient sees)
+// exp2 (a few FP32 ulp: below the FP32 rounding of theta_v that its gradient sees).
+// theta_v enters only N^2 = grad theta_v . grad Phi / theta_v: without a gravitational
+// potential (buoy false: grad Phi = 0, N^2 = 0 exactly) it is not evaluated and the
+// argument holds 1 in its place.
 template <bool SMAG>
 __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M, const double q[5], const ThermoD &t,
-                                           float G[5]) {
+                                           bool buoy, float G[5]) {
 #pragma clang fp contract(fast)
     const double rinv = 1.0 / q[0];
     G[0] = (float)(rinv * q[1]);
     G[1] = (float)(rinv * q[2]);
     G[2] = (float)(rinv * q[3]);
     G[3] = (float)(q[4] * rinv + P.R_d * t.temp);
-    G[4] = SMAG ? (float)t.temp *
-                      __builtin_amdgcn_exp2f(-M.kappa * __builtin_amdgcn_logf((float)t.p * M.inv_MSLP))
-                : 0.0f;
+    G[4] = !SMAG ? 0.0f
+           : buoy ? (float)t.temp * __builtin_amdgcn_exp2f(-M.kappa * __builtin_amdgcn_logf((float)t.p * M.inv_MSLP))
+                  : 1.0f;
 }
 
 // ------------------------------------------------------------------ gradient (mixed)
 // Five waves per SIMD: the Smagorinsky-Lilly variant waits on memory more than it
 // computes (the learned one is VALU-bound), and the register cap this asks for (96)
 // spills nothing at Nq = 5.
-template <int Nq, bool SMAG, bool LEARN>
+// BUOY: the context has a gravitational potential (FlatOrientation), so Smagorinsky-Lilly's
+// N^2 = grad theta_v . grad Phi / theta_v needs theta_v and its gradient; without one
+// (grad Phi = 0, N^2 = 0 exactly) the kernel evaluates neither.
+template <int Nq, bool SMAG, bool LEARN, bool BUOY>
 __global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(5)))
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
                double *__restrict__ aux, float *__restrict__ gfo,
@@ -163,6 +169,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     const int64_t e = set_element(ES, blockIdx.x);
     if (e < 0) return;
     const int64_t ce = static_class(A, e);
+    constexpr bool buoy = SMAG && BUOY;
     __shared__ int32_t nb[6];  // the element's face neighbours
     if (threadIdx.x < 6) nb[threadIdx.x] = A.nbe[6 * e + threadIdx.x];
     const int n = threadIdx.x;
@@ -186,7 +193,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             gPhi[2] = gp[2 * Np];
         }
         const ThermoD t = thermo_d(P, q, Phi);
-        grad_arg_f<SMAG>(P, MP, q, t, G);
+        grad_arg_f<SMAG>(P, MP, q, t, buoy, G);
         thv = SMAG ? G[4] : 1.0f;
         if (write_aux) {
             double *aw = aux + (int64_t)Np * P.naux * e + n;
@@ -216,7 +223,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             for (int s = 0; s < 5; ++s) qp[s] = Q[qb + Np * s];
             const double Php = A.aux_cont ? own_aux<Nq>(P, A.sa, ce, vn, P.a_Phi, -0.0)
                                           : A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
-            grad_arg_f<SMAG>(P, MP, qp, thermo_d(P, qp, Php), Gp);
+            grad_arg_f<SMAG>(P, MP, qp, thermo_d(P, qp, Php), buoy, Gp);
 #pragma unroll
             for (int s = 0; s < 5; ++s) jump[s] = nd * (0.5f * (Gp[s] - Gm[s]));
         } else {
@@ -226,7 +233,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 if (c == d) qb[1 + c] = 0.0;  // the gradient wall state removes the normal momentum
-            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, own_aux<Nq>(P, A.sa, ce, vn, P.a_Phi, -0.0)), Gp);
+            grad_arg_f<SMAG>(P, MP, qb, thermo_d(P, qb, own_aux<Nq>(P, A.sa, ce, vn, P.a_Phi, -0.0)), buoy, Gp);
 #pragma unroll
             for (int s = 0; s < 5; ++s) jump[s] = nd * (Gp[s] - Gm[s]);
         }
@@ -242,7 +249,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
             float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
-            if (s < NG) {
+            if (s < NG && (s < 4 || buoy)) {
 #pragma unroll
                 for (int m = 0; m < Nq; ++m) {
                     g1 += sD[i + Nq * m] * sG[s][m + Nq * (j + Nq * k)];

@@ -37,6 +37,16 @@ def _case(name, N=4):
         Q0[:, 0, :] *= 1 + 1e-4 * rng.uniform(-1, 1, Q0[:, 0, :].shape)
         Q0[:, 1:4, :] += 0.1 * rng.uniform(-1, 1, Q0[:, 1:4, :].shape)
         return cfg, grid, aux, Q0
+    if name == "tgv_smagorinsky":
+        # Taylor-Green vortex, Smagorinsky-Lilly without orientation (N^2 = 0): the plain
+        # method of the 96^3 workload
+        prm = atmos.ideal_gas_parameters()
+        grid = uniform_brick((3, 3, 3), (0.0, 0.0, 0.0), (2 * math.pi,) * 3, N=N)
+        cfg = atmos.AtmosConfig(prm, turbulence=abi.SMAGORINSKY_LILLY, C_smag=0.21)
+        aux = atmos.init_aux(cfg, grid)
+        Q0 = atmos.taylor_green(prm, grid, mach=0.1)
+        Q0[:, 1:4, :] *= 1 + 0.05 * rng.uniform(-1, 1, Q0[:, 1:4, :].shape)
+        return cfg, grid, aux, Q0
     if name == "tgv_kinematic":
         # Taylor-Green vortex, constant kinematic viscosity with the divergence term
         prm = atmos.ideal_gas_parameters()
@@ -64,8 +74,8 @@ def _case(name, N=4):
     return cfg, grid, aux, Q0
 
 
-CASES = ["euler_periodic", "euler_central", "viscous_periodic", "tgv_kinematic", "les_walls", "les_central",
-         "les_no_subtract"]
+CASES = ["euler_periodic", "euler_central", "viscous_periodic", "tgv_kinematic", "tgv_smagorinsky", "les_walls",
+         "les_central", "les_no_subtract"]
 
 
 def _model(cfg, grid, aux, geometry):
@@ -124,13 +134,13 @@ def test_rhs_parity(name, geometry):
 
 
 @pytest.mark.parametrize("geometry", ["affine", "general"])
-@pytest.mark.parametrize("name", ["euler_periodic", "tgv_kinematic", "les_walls"])
+@pytest.mark.parametrize("name", ["euler_periodic", "tgv_kinematic", "tgv_smagorinsky", "les_walls"])
 def test_lsrk54_step_parity(name, geometry):
     import torch
     from cmdg.dg import LSRK54CarpenterKennedy, solve, to_device, to_host
     print(f"[parity] lsrk54 {name} {geometry}", flush=True)
     cfg, grid, aux, Q0 = _case(name)
-    dt = {"euler_periodic": 1e-3, "tgv_kinematic": 1e-4}.get(name, 5e-3)
+    dt = {"euler_periodic": 1e-3, "tgv_kinematic": 1e-4, "tgv_smagorinsky": 1e-4}.get(name, 5e-3)
     orc = O.Oracle(cfg, grid, aux)
     Qr = np.asfortranarray(Q0.copy())
     dQr = np.zeros_like(Qr, order="F")
@@ -223,7 +233,7 @@ def test_update_kernel_order4():
 MIXED_RHS_RTOL = 1e-5   # FP32 fluxes/derivatives vs the FP64 oracle; measured max 2.3e-6 (mass, energy)
 
 
-@pytest.mark.parametrize("name", ["euler_periodic", "viscous_periodic", "les_walls", "les_central"])
+@pytest.mark.parametrize("name", ["euler_periodic", "viscous_periodic", "tgv_smagorinsky", "les_walls", "les_central"])
 def test_mixed_rhs_error(name):
     """CMDG_MIXED right-hand side vs the FP64 oracle: relative L2 error per component."""
     import torch
