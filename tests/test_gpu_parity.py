@@ -169,21 +169,28 @@ def test_lsrk54_step_parity(name, geometry):
     dg.close()
 
 
+# bound on the mixed path's distance from the Float64 known answer (relative): its state,
+# register and stage update stay FP64, so it lands far closer to the Float64 value than
+# the all-Float32 reference run (1.8e-5 away)
+MIXED_VORTEX_RTOL = 2e-6
+
+
 @pytest.mark.parametrize("variant", ["fp64_rusanov", "fp64_central", "mixed_rusanov"])
 def test_vortex_known_answer_on_gpu(variant):
     """Isentropic vortex, 3-D, level 1, run entirely through the HIP library, against the
     reference's expected_error (isentropicvortex.jl, table and line in
-    tests/golden/reference_kats.json) with its tolerance sqrt(eps(FT)): FP64 Rusanov
-    (:105) and Central (:110); the mixed-precision path against the reference's Float32
-    Rusanov value (:195) with the Float32 tolerance."""
+    tests/golden/reference_kats.json) with its tolerance sqrt(eps(Float64)): FP64 Rusanov
+    (:105) and Central (:110).  The mixed-precision path is held to the Float64 Rusanov
+    value (:105) within MIXED_VORTEX_RTOL, and must be at least 5x closer to it than the
+    reference's all-Float32 value (:195) is."""
     import json
     from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
     print(f"[parity] vortex known answer {variant}", flush=True)
     kats = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
     table = {"fp64_rusanov": "Float64 Rusanov", "fp64_central": "Float64 Central",
-             "mixed_rusanov": "Float32 Rusanov"}[variant]
+             "mixed_rusanov": "Float64 Rusanov"}[variant]
     expected, _line = kats["isentropic_vortex_3d"]["expected_error"][table]["1"]
-    rtol = math.sqrt(np.finfo(np.float32 if variant.startswith("mixed") else np.float64).eps)
+    rtol = MIXED_VORTEX_RTOL if variant.startswith("mixed") else math.sqrt(np.finfo(np.float64).eps)
     prm = atmos.earth_parameters()
     setup = atmos.IsentropicVortex()
     L = setup.domain_halflength
@@ -198,8 +205,11 @@ def test_vortex_known_answer_on_gpu(variant):
     solver = LSRK54CarpenterKennedy(dg, Q, dt=dt, t0=0.0)
     solve(Q, solver, timeend=timeend)
     err = O.euclidean_distance(to_host(Q), setup(prm, grid, timeend), grid.mass)
-    print(f"  error {err!r} (expected {expected!r})", flush=True)
+    print(f"  error {err!r} (expected {expected!r}, relative distance {abs(err / expected - 1):.2e})", flush=True)
     assert err == pytest.approx(expected, rel=rtol)
+    if variant.startswith("mixed"):
+        f32, _ = kats["isentropic_vortex_3d"]["expected_error"]["Float32 Rusanov"]["1"]
+        assert abs(err - expected) < 0.2 * abs(f32 - expected)
     dg.close()
 
 
