@@ -14,7 +14,7 @@ LSRK54CarpenterKennedy step = 5 stages.  Three legs from the same initial state:
   * mixed precision with Smagorinsky-Lilly: the error of the precision change alone.
 --workload les25 runs the dry rising-bubble LES on 25^3 elements instead (gravity,
 hydrostatic reference, walls in z; the network there has the constant output
-2 log C_s, Smagorinsky-Lilly's nu_0 through the fused MLP).
+C_s^2, Smagorinsky-Lilly's nu_0 through the fused MLP).
 
 Timed region: K steps as one solve!(Q, solver; numberofsteps = K) without callbacks,
 state already resident in HBM, bracketed by device synchronisation.  GDOF/s per
@@ -107,15 +107,15 @@ def build_tgv(ne=96, lean=True):
 
 def closure_weights(workload: str):
     """The network the chosen form runs: on the Taylor-Green vortex the one trained on
-    the filtered 64^3 run; on the LES bubble the network whose output is 2 log C_s
+    the filtered 64^3 run; on the LES bubble the network whose output is C_s^2
     everywhere (Smagorinsky-Lilly's nu_0 through the fused MLP: the state difference to
     the plain run then measures the arithmetic of the chosen form alone; the MLP's cost
     does not depend on its weights)."""
     from cmdg import closure
     if workload == "tgv96":
         return closure.MLPParams.load(WEIGHTS), "trained on the box-filtered 64^3 Re 1600 Taylor-Green run"
-    return closure.MLPParams.constant(2 * math.log(0.23)), \
-        "network with constant output 2 log C_s (Smagorinsky-Lilly's nu_0 through the fused MLP)"
+    return closure.MLPParams.constant(0.23 ** 2), \
+        "network with constant output C_s^2 (Smagorinsky-Lilly's nu_0 through the fused MLP)"
 
 
 def cpu_model() -> str:

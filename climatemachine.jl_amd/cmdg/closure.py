@@ -3,10 +3,12 @@
 The closure replaces nu_0 of SmagorinskyLilly
 (src/Common/TurbulenceClosures/TurbulenceClosures.jl:478-491) by
 
-    nu_0 = exp(min(y, -1)) * Delta^2 * |S| + nu_floor,   y = MLP((raw - shift) * scale),
+    nu_0 = max(nu_floor + clamp(y, -e^-1, e^-1) * Delta^2 * |S|, 0),   y = MLP((raw - shift) * scale),
 
 raw = (du_c/dx_d at 3c + d, Delta), a 10 -> 32 -> 32 -> 1 network with SiLU
-hidden units, nu_floor the context's kinematic viscosity (1e-5 when zero).  The right-hand side evaluates it with BF16 MFMA tiles fused into
+hidden units, nu_floor the context's kinematic viscosity (1e-5 when zero): a signed
+eddy-viscosity coefficient, whose negative values take viscosity away (backscatter)
+down to none in total.  The right-hand side evaluates it with BF16 MFMA tiles fused into
 the mixed-precision gradient kernel; this module holds the FP32 master
 parameters, exports them as BF16, and drives the library's training, filter,
 sample and spectrum kernels.  It has no counterpart in the reference except the
@@ -30,7 +32,7 @@ NPARAM = abi.MLP_NPARAM
 OFF_W1, OFF_B1, OFF_W2, OFF_B2, OFF_W3, OFF_B3 = 0, 320, 352, 1376, 1408, 1440
 NIN, NH = 10, 32
 SAMPLE_W = 13        # raw inputs (10), then P, Q, R of the stress loss
-YMAX = -1.0          # output clamp: nu_0 <= 0.37 Delta^2 |S| (C_s <= 0.6)
+CMAX = math.exp(-1.0)  # output clamp: |nu_0 - nu_floor| <= 0.37 Delta^2 |S| (|C_s| <= 0.6)
 NMOM = 11            # filtered fields: rho, rho u (3), rho e, rho u_a u_b (6)
 
 
@@ -90,8 +92,9 @@ class MLPParams:
 
     @classmethod
     def constant(cls, y0: float) -> "MLPParams":
-        """A network whose output is y0 everywhere: with y0 = 2 log C_s it is
-        Smagorinsky-Lilly's nu_0 = |S| (C_s Delta)^2 + 1e-5."""
+        """A network whose output is y0 everywhere: with y0 = C_s^2 (and the floor 1e-5
+        of a context without viscosity) it is Smagorinsky-Lilly's
+        nu_0 = |S| (C_s Delta)^2 + 1e-5."""
         flat = np.zeros(NPARAM, np.float32)
         flat[OFF_B3] = y0
         return cls(flat)

@@ -86,7 +86,7 @@ def main(argv=None):
     q0 = to_device(Q0).cpu()
     legs = [("mixed", cfg, None)]
     if a.learned:
-        legs.append(("mixed_learned_const", cfgl, closure.MLPParams.constant(2 * math.log(0.23))))
+        legs.append(("mixed_learned_const", cfgl, closure.MLPParams.constant(0.23 ** 2)))
     for name, c, mlp in legs:
         dg = DGModel(c, grid, aux=aux, precision=abi.MIXED)
         if mlp is not None:

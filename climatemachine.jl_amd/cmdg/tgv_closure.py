@@ -39,21 +39,23 @@ def log(msg: str):
 
 
 def setup(n: int, mach: float, re: float, L: float = 1.0):
+    """The periodic (2 pi L)^3 brick of n^3 elements (lean: the library builds its element
+    records from the break points, and nothing here reads per-node geometry)."""
     prm = atmos.ideal_gas_parameters()
-    grid = uniform_brick((n, n, n), (0, 0, 0), (2 * math.pi * L,) * 3, N=4)
+    grid = uniform_brick((n, n, n), (0, 0, 0), (2 * math.pi * L,) * 3, N=4, lean=True)
     c0 = math.sqrt(prm.cp_d / prm.cv_d * prm.R_d * 300.0)
     U0 = mach * c0
     return prm, grid, U0, U0 * L / re
 
 
 def model(prm, grid, kind: str, nu: float, mlp=None, C_smag: float = 0.21):
-    """kind: fine / none (constant kinematic viscosity nu), smag / smag64 (reference
-    Smagorinsky-Lilly, mixed / FP64), net (the learned closure, mixed precision, with
-    nu as its viscosity floor, running network `mlp`)."""
-    if kind in ("fine", "none"):
+    """kind: fine / none (constant kinematic viscosity nu, FP64; fine_mixed the same in mixed
+    precision), smag / smag64 (reference Smagorinsky-Lilly, mixed / FP64), net (the learned
+    closure, mixed precision, with nu as its viscosity floor, running network `mlp`)."""
+    if kind in ("fine", "none", "fine_mixed"):
         cfg = atmos.AtmosConfig(prm, turbulence=abi.CONSTANT_KINEMATIC_VISCOSITY, viscosity=nu,
                                 with_divergence=True)
-        return DGModel(cfg, grid, precision=abi.FP64)
+        return DGModel(cfg, grid, precision=abi.MIXED if kind == "fine_mixed" else abi.FP64)
     if kind in ("smag", "smag64"):
         cfg = atmos.AtmosConfig(prm, turbulence=abi.SMAGORINSKY_LILLY, C_smag=C_smag)
         return DGModel(cfg, grid, precision=abi.MIXED if kind == "smag" else abi.FP64)
@@ -147,7 +149,7 @@ def stage_train(a):
     for v, table in tables.items():
         tab = table[:, :n]
         rms = float(torch.sqrt((tab[:9].double() ** 2).mean()).item())
-        mlp0 = closure.MLPParams.random(seed=a.seed, y0=2 * math.log(0.1),
+        mlp0 = closure.MLPParams.random(seed=a.seed, y0=0.0,
                                         shift=np.zeros(10), scale=np.r_[np.full(9, 1.0 / rms), 1.0 / Delta])
         P, Qv, R = (tab[10].double(), tab[11].double(), tab[12].double())
         c2 = float((-(Qv.sum()) / P.sum()).item())
@@ -188,40 +190,80 @@ def stage_eval(a):
     for path in (a.weights.split(",") if a.weights else []):
         nets["learned_" + os.path.splitext(os.path.basename(path))[0].replace("closure_", "")] = \
             closure.MLPParams.load(path)
+    for name in list(nets):
+        for alpha in ([float(x) for x in a.scales.split(",")] if a.scales else []):
+            # the network's output times alpha (its last layer scaled): a candidate of the
+            # a-posteriori calibration against the filtered fine run
+            m = closure.MLPParams(nets[name].flat, nets[name].shift, nets[name].scale)
+            m.flat[closure.OFF_W3:closure.OFF_B3 + 1] *= alpha
+            nets[f"{name}_x{alpha:g}"] = m
     if a.fitted_cs > 0:
-        nets["fitted_const"] = closure.MLPParams.constant(2 * math.log(a.fitted_cs))
-    nets["smag021_floor"] = closure.MLPParams.constant(2 * math.log(0.21))
+        nets["fitted_const"] = closure.MLPParams.constant(a.fitted_cs ** 2)
+    if "smag_floor" in a.models.split(","):
+        nets["smag021_floor"] = closure.MLPParams.constant(0.21 ** 2)
     dt_f = time_step(fine, U0, a.courant, prm)
-    tend = a.tend / U0
-    nf = int(math.ceil(tend / dt_f))
-    dt_f = tend / nf
-    nc = nf // 2
-    dt_c = tend / nc
-    times = [float(x) for x in a.eval_times.split(",")]
+    if a.coarse_steps > 0:
+        # a run of a.coarse_steps coarse steps (twice as many fine steps at the fine mesh's
+        # Courant step); --eval-times are coarse step counts, reported in t*
+        nc = a.coarse_steps
+        nf = 2 * nc
+        tend = nf * dt_f
+        a.tend = tend * U0
+        dt_c = tend / nc
+        times = [int(x) * dt_c * U0 for x in a.eval_times.split(",")]
+    else:
+        tend = a.tend / U0
+        nf = int(math.ceil(tend / dt_f))
+        dt_f = tend / nf
+        nc = nf // 2
+        dt_c = tend / nc
+        times = [float(x) for x in a.eval_times.split(",")]
     marks_f = {int(round(t / a.tend * nf)): t for t in times}
     marks_c = {int(round(t / a.tend * nc)): t for t in times}
     p = a.points
     log(f"[eval] fine {a.fine}^3 ({nf} steps), coarse {a.coarse}^3 ({nc} steps), t* {times}")
     Q0f = atmos.taylor_green(prm, fine, mach=a.mach)
-    dg = model(prm, fine, "fine", nu)
     Q = to_device(Q0f)
-    ref = {}
+    del Q0f
     Qc0 = closure.box_filter(Q, fine, coarse)[:, :5].contiguous()
+    ref = {}
+    if a.reference:
+        # the filtered fine run's figures from an earlier --save-reference of the same setup
+        with open(a.reference) as f:
+            stored = json.load(f)
+        for key in ("fine", "coarse", "mach", "re", "tend", "courant", "points", "fine_precision"):
+            saved = stored.get(key, "fp64" if key == "fine_precision" else None)
+            if saved != getattr(a, key):
+                raise SystemExit(f"--reference was saved with {key} = {saved}, not {getattr(a, key)}")
+        for t in times:
+            r = stored["times"][str(t)]
+            ref[t] = (np.asarray(r["spectrum"]), r["ke"])
+        el_f = stored["fine_seconds"]
+    else:
+        dg = model(prm, fine, "fine_mixed" if a.fine_precision == "mixed" else "fine", nu)
 
-    def snap_f(step, t):
-        if step in marks_f:
-            Qc = closure.box_filter(Q, fine, coarse)[:, :5].contiguous()
-            E, k, ke = spectrum(Qc, coarse, p, 1.0)
-            ref[marks_f[step]] = (E, ke)
-            log(f"[eval] reference t*={marks_f[step]}: KE {ke:.5e}")
-    gcd = math.gcd(*[s for s in marks_f]) if marks_f else 1
-    el_f = run(dg, Q, dt_f, nf, snap_f, gcd, "fine")
-    dg.close()
+        def snap_f(step, t):
+            if step in marks_f:
+                Qc = closure.box_filter(Q, fine, coarse)[:, :5].contiguous()
+                E, k, ke = spectrum(Qc, coarse, p, 1.0)
+                ref[marks_f[step]] = (E, ke)
+                log(f"[eval] reference t*={marks_f[step]}: KE {ke:.5e}")
+        gcd = math.gcd(*[s for s in marks_f]) if marks_f else 1
+        el_f = run(dg, Q, dt_f, nf, snap_f, gcd, "fine")
+        dg.close()
+        if a.save_reference:
+            with open(a.save_reference, "w") as f:
+                json.dump(dict(fine=a.fine, coarse=a.coarse, mach=a.mach, re=a.re, tend=a.tend, courant=a.courant,
+                               points=p, fine_precision=a.fine_precision, steps_fine=nf, fine_seconds=el_f,
+                               times={str(t): dict(ke=ref[t][1], spectrum=[float(x) for x in ref[t][0]])
+                                      for t in times}), f, indent=1)
     del Q
     torch.cuda.empty_cache()
     out = dict(stage="eval", fine=a.fine, coarse=a.coarse, U0=U0, nu=nu, tend=a.tend, dt_fine=dt_f,
                steps_fine=nf, steps_coarse=nc, fine_seconds=el_f, points_per_element=p, models={})
-    kinds = [k for k in a.models.split(",") if k != "nets"] + (list(nets) if "nets" in a.models.split(",") else [])
+    kinds = [k for k in a.models.split(",") if k and k not in ("nets", "smag_floor")] + \
+        ([k for k in nets if k != "smag021_floor"] if "nets" in a.models.split(",") else []) + \
+        (["smag021_floor"] if "smag021_floor" in nets else [])
     for kind in kinds:
         dgc = model(prm, coarse, "net", nu, nets[kind]) if kind in nets else model(prm, coarse, kind, nu)
         Qc = Qc0.clone()
@@ -274,10 +316,22 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--weights", default=None)
     ap.add_argument("--models", default="nets,smag,smag64,none",
-                    help="'nets' = every network given (--weights, --fitted-cs) and Smagorinsky C_s = 0.21 "
-                         "with the molecular floor")
+                    help="'nets' = every network given (--weights, --fitted-cs); 'smag_floor' = the constant "
+                         "network of Smagorinsky C_s = 0.21 on top of the molecular floor")
     ap.add_argument("--eval-times", default="4,6,8")
     ap.add_argument("--points", type=int, default=4)
+    ap.add_argument("--scales", default=None,
+                    help="eval: also run every network with its output multiplied by each of these factors")
+    ap.add_argument("--coarse-steps", type=int, default=0,
+                    help="eval: run this many coarse steps (2x as many fine ones) instead of --tend; "
+                         "--eval-times are then coarse step counts")
+    ap.add_argument("--fine-precision", choices=("fp64", "mixed"), default="fp64",
+                    help="eval: precision of the fine reference run (constant kinematic viscosity)")
+    ap.add_argument("--save-reference", default=None,
+                    help="eval: write the filtered fine run's kinetic energy and spectra at the evaluation times here")
+    ap.add_argument("--reference", default=None,
+                    help="eval: take the filtered fine run's figures from this file (written by --save-reference "
+                         "with the same setup) instead of running it")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     if not torch.cuda.is_available():

@@ -936,20 +936,7 @@ int cmdg_mlp_set_weights(cmdg_ctx *c, const uint16_t *w1, const float *b1, const
     if (!c) return CMDG_EINVAL;
     if (!c->learned) return fail(c, CMDG_EINVAL, "the context was not created with the learned closure");
     if (!w1 || !b1 || !w2 || !b2 || !w3 || !b3 || !shift || !scale) return fail(c, CMDG_EINVAL, "null pointer");
-    MlpDev h{};
-    for (int o = 0; o < MLP_H; ++o) {
-        for (int k = 0; k < MLP_IN; ++k) h.w1[o * MLP_K1 + k] = w1[o * MLP_IN + k];
-        for (int k = 0; k < MLP_H; ++k) h.w2[o * MLP_H + k] = w2[o * MLP_H + k];
-        h.b1[o] = b1[o];
-        h.b2[o] = b2[o];
-        const uint32_t bits = (uint32_t)w3[o] << 16;
-        std::memcpy(&h.w3[o], &bits, 4);
-    }
-    h.b3 = b3[0];
-    for (int k = 0; k < MLP_IN; ++k) {
-        h.shift[k] = shift[k];
-        h.scale[k] = scale[k];
-    }
+    const MlpDev h = mlp_device_copy(w1, b1, w2, b2, w3, b3, shift, scale);
     int rc = check_hip(c, hipMemcpyAsync(c->d_mlp, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
     if (rc) return rc;
     rc = check_hip(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize");

@@ -141,10 +141,11 @@ k_box_filter(const double *__restrict__ Qf, const int32_t *__restrict__ children
 // One sample per coarse node from the filtered moments M (Np, 11, E) and the coarse
 // velocity gradient Gu (Np, 9, E):
 //   u = (rho u)/rho,  tau_ab = (rho u_a u_b)/rho - u_a u_b   (subgrid stress),
-//   S = sym(Gu), S^d its deviator, |S| = sqrt(2 S:S), c = Delta^2 |S|,
+//   S = sym(Gu), S^d its deviator, |S| = sqrt(2 S:S), k = Delta^2 |S|,
 //   T = tau^d - 2 (nu_mol - nu_floor) S^d   (what nu_0 - nu_floor must supply),
-//   a = 2 c S^d;  the model's deviatoric stress is -exp(y) a - 2 nu_floor S^d, so
-//   the squared error is exp(2y) P + 2 exp(y) Q + R with P = a:a, Q = a:T, R = T:T.
+//   a = 2 k S^d;  with the network's coefficient c the model's deviatoric stress is
+//   -c a - 2 nu_floor S^d, so the squared error is c^2 P + 2 c Q + R with P = a:a,
+//   Q = a:T, R = T:T.
 // Row layout (SoA, leading dimension ld): rows 0-8 du_c/dx_d at 3c + d, row 9
 // Delta, rows 10-12 P, Q, R.
 constexpr int SAMPLE_W = 13;
@@ -229,7 +230,7 @@ __device__ __forceinline__ float silu_fwd(float a, float &dsilu) {
 // Per workgroup of 64 samples: the loss sum and its gradient with respect to every
 // parameter, written to partial[blockIdx][GRAD_W].  Forward as the inference (FP32
 // here, with the same input normalisation and output clamp), loss
-// sum_s (exp(2y)P + 2 exp(y)Q + R) * inv_norm.
+// sum_s (c^2 P + 2 c Q + R) * inv_norm, c = clamp(y, MLP_CMIN, MLP_CMAX).
 __global__ void __launch_bounds__(TRAIN_TB)
 k_mlp_grad(const float *__restrict__ params, const float *__restrict__ shift, const float *__restrict__ scale,
            const float *__restrict__ samples, int64_t ld, const int64_t *__restrict__ idx, int64_t batch,
@@ -271,10 +272,10 @@ k_mlp_grad(const float *__restrict__ params, const float *__restrict__ shift, co
     }
     float dy = 0.0f, L = 0.0f;
     if (valid) {
-        const bool clamped = y > MLP_YMAX;
-        const float ex = expf(clamped ? MLP_YMAX : y);
-        L = (ex * ex * P + 2.0f * ex * Qv + R) * inv_norm;
-        dy = clamped ? 0.0f : (2.0f * ex * ex * P + 2.0f * ex * Qv) * inv_norm;
+        const bool clamped = y > MLP_CMAX || y < MLP_CMIN;
+        const float c = fminf(fmaxf(y, MLP_CMIN), MLP_CMAX);
+        L = (c * c * P + 2.0f * c * Qv + R) * inv_norm;
+        dy = clamped ? 0.0f : (2.0f * c * P + 2.0f * Qv) * inv_norm;
     }
     sDy[t] = dy;
     sL[t] = L;

@@ -188,12 +188,13 @@ int cmdg_gradient_flux_f32(cmdg_ctx *ctx, float **gradflux_device, int32_t *ngra
  * SmagorinskyLilly, src/Common/TurbulenceClosures/TurbulenceClosures.jl:478-491).
  * A per-node MLP 10 -> 32 -> 32 -> 1 with SiLU hidden units.  Raw inputs: the
  * velocity gradient du_c/dx_d at index 3c + d and the filter width Delta at 9;
- * the network sees x = (raw - shift) * scale; its output y gives
- *     nu_0 = exp(min(y, -1)) * Delta^2 * |S| + nu_floor,   |S| = sqrt(2 S:S)
- * (the clamp keeps the network's part below 0.37 Delta^2 |S|, a Smagorinsky
- * coefficient of 0.6); nu_floor is the descriptor's `viscosity` (the molecular
- * kinematic viscosity), 1e-5 when that is zero, so that a constant output 2 log C_s
- * with viscosity 0 is Smagorinsky-Lilly.
+ * the network sees x = (raw - shift) * scale; its output y, a signed coefficient, gives
+ *     nu_0 = max(nu_floor + clamp(y, -e^-1, e^-1) * Delta^2 * |S|, 0),   |S| = sqrt(2 S:S)
+ * (the clamp keeps the network's part within 0.37 Delta^2 |S| either way, a
+ * Smagorinsky coefficient of 0.6; a negative y takes viscosity away, backscatter, never
+ * below zero in total); nu_floor is the descriptor's `viscosity` (the molecular
+ * kinematic viscosity), 1e-5 when that is zero, so that a constant output C_s^2 with
+ * viscosity 0 is Smagorinsky-Lilly.
  * Weights are BF16 (row-major [out][in]: w1 32x10, w2 32x32, w3 1x32), biases FP32
  * (b1 32, b2 32, b3 1), shift/scale 10 FP32 each; all host pointers.  Evaluated
  * with BF16 MFMA / FP32 accumulation inside the mixed-precision gradient kernel. */
@@ -222,7 +223,7 @@ int cmdg_box_filter(const double *Q_fine, const int32_t *children, int64_t ncoar
 /* Training samples of the subgrid stress, one per coarse node, SoA with leading
  * dimension ld starting at column `offset`: rows 0-8 raw velocity gradient, row 9
  * Delta, rows 10-12 P, Q, R such that the squared deviatoric-stress error of the
- * closure is exp(2y) P + 2 exp(y) Q + R (the stress the network must supply is the
+ * closure is c^2 P + 2 c Q + R, c = clamp(y, -e^-1, e^-1) (the stress the network must supply is the
  * subgrid stress plus the molecular stress 2 (nu_mol - nu_floor) S^d that the
  * closure's floor nu_floor does not already give).  element_average != 0 replaces
  * P, Q, R by their mean over each element (locally averaged targets). */
@@ -233,7 +234,7 @@ int cmdg_closure_samples(const double *moments, const double *grad_u, int64_t ne
 /* MLP training, FP32 master parameters (CMDG_MLP_NPARAM, flat: W1[32][10], b1[32],
  * W2[32][32], b2[32], W3[32], b3).  Loss and gradient over a batch of samples
  * (rows idx[0..batch) of the sample table, or its first `batch` columns when idx is
- * NULL): grad[0..NPARAM) and grad[NPARAM] = loss = sum (exp(2y)P + 2exp(y)Q + R) *
+ * NULL): grad[0..NPARAM) and grad[NPARAM] = loss = sum (c^2 P + 2 c Q + R) *
  * inv_norm.  work: device scratch of at least ceil(batch/64) * (NPARAM+1) floats. */
 #define CMDG_MLP_NPARAM 1441
 int cmdg_mlp_loss_grad(const float *params, const float *shift, const float *scale,

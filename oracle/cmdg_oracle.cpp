@@ -16,10 +16,11 @@
 // and Smagorinsky-Lilly closure on linear velocity fields.  Smagorinsky-Lilly
 // has no reference known-answer value; the closed-form test is its only pin.
 // The learned closure (no reference counterpart) replaces Smagorinsky-Lilly's
-// nu_0 by exp(min(y, -1)) Delta^2 |S| + nu_floor (the kinematic viscosity of the
-// descriptor, 1e-5 when zero), y the output of a 10-32-32-1 SiLU network on
-// the assembled velocity gradient; its inference here emulates the BF16 rounding
-// of the GPU (inputs, weights, first hidden layer) with exact arithmetic otherwise.
+// nu_0 by max(nu_floor + clamp(y, -e^-1, e^-1) Delta^2 |S|, 0) (nu_floor the kinematic
+// viscosity of the descriptor, 1e-5 when zero), y the output of a 10-32-32-1 SiLU
+// network on the assembled velocity gradient; its inference here emulates the GPU's
+// roundings (BF16 inputs, weights and first hidden layer; the FP32 base-2 parameters of
+// the hidden layers) with exact arithmetic otherwise.
 // Pinned by: a constant-output network reproduces Smagorinsky-Lilly, and the
 // network against an independent numpy evaluation (tests/test_closure_oracle.py).
 #include "../include/cmdg.h"
@@ -80,18 +81,21 @@ inline double bf16_round(double v) {
 }
 
 // Network output for raw inputs (9 velocity-gradient entries du_c/dx_d at 3c+d, Delta).
+// The hidden layers run in base-2 units as on the GPU: u = -log2(e) a, and a unit's
+// output u / (1 + 2^u) = -SiLU(a) / ln 2, with scale, b1, b2 taken times -log2(e) and w3
+// times -ln 2 (each rounded to FP32, oracle_mlp_set).
 template <class T> double mlp_output(const Ctx<T> &c, const double raw[10]) {
     double x[10], h1[32], y = c.b3;
-    for (int k = 0; k < 10; ++k) x[k] = bf16_round((double)(float)(((float)raw[k] - (float)c.shift[k]) * (float)c.scale[k]));
+    for (int k = 0; k < 10; ++k) x[k] = bf16_round((double)(((float)raw[k] - (float)c.shift[k]) * (float)c.scale[k]));
     for (int o = 0; o < 32; ++o) {
-        double a = c.b1[o];
-        for (int k = 0; k < 10; ++k) a += c.w1[10 * o + k] * x[k];
-        h1[o] = bf16_round(a / (1.0 + std::exp(-a)));
+        double u = c.b1[o];
+        for (int k = 0; k < 10; ++k) u += c.w1[10 * o + k] * x[k];
+        h1[o] = bf16_round(u / (1.0 + std::exp2(u)));
     }
     for (int o = 0; o < 32; ++o) {
-        double a = c.b2[o];
-        for (int k = 0; k < 32; ++k) a += c.w2[32 * o + k] * h1[k];
-        y += c.w3[o] * (a / (1.0 + std::exp(-a)));
+        double u = c.b2[o];
+        for (int k = 0; k < 32; ++k) u += c.w2[32 * o + k] * h1[k];
+        y += c.w3[o] * (u / (1.0 + std::exp2(u)));
     }
     return y;
 }
@@ -423,12 +427,13 @@ template <class T> void learned_viscosity(Ctx<T> &c, const T *aux) {
             double raw[10];
             for (int s = 0; s < 9; ++s) raw[s] = (double)at(10 + s);
             raw[9] = (double)a.Delta;
-            const double y = std::min(mlp_output(c, raw), -1.0);
+            const double cmax = 0.36787944117144233;  // e^-1
+            const double y = std::min(std::max(mlp_output(c, raw), -cmax), cmax);
             const double S0 = at(3), S1 = at(4), S2 = at(5), S3 = at(6), S4 = at(7), S5 = at(8);
             const double norm2 = S0 * S0 + S3 * S3 + S5 * S5 + 2.0 * (S1 * S1 + S2 * S2 + S4 * S4);
             const double D = (double)a.Delta;
             const double floor = c.visc > T(0) ? (double)(float)c.visc : (double)1e-5f;
-            at(19) = (T)(std::exp(y) * (D * D) * std::sqrt(2.0 * norm2) + floor);
+            at(19) = (T)std::max(floor + y * (D * D) * std::sqrt(2.0 * norm2), 0.0);
         }
 }
 
@@ -874,21 +879,24 @@ int oracle_flux_second_order(oracle_ctx *o, const double *q, const double *gf, c
     return 0;
 }
 
-// Learned-closure network (BF16-valued weights passed as double; shift/scale FP32).
+// Learned-closure network (BF16-valued weights passed as double; biases, shift and scale
+// FP32), kept in the base-2 units mlp_output uses: scale, b1, b2 times -log2(e), w3 (its
+// BF16 value) times -ln 2, each rounded to FP32, as the GPU's device copy holds them.
 int oracle_mlp_set(oracle_ctx *o, const double *w1, const double *b1, const double *w2, const double *b2,
                    const double *w3, double b3, const double *shift, const double *scale) {
+    const double log2e = 1.4426950408889634, ln2 = 0.6931471805599453;
     auto set = [&](auto &c) {
         for (int q = 0; q < 320; ++q) c.w1[q] = bf16_round(w1[q]);
         for (int q = 0; q < 1024; ++q) c.w2[q] = bf16_round(w2[q]);
         for (int q = 0; q < 32; ++q) {
-            c.w3[q] = bf16_round(w3[q]);
-            c.b1[q] = (double)(float)b1[q];
-            c.b2[q] = (double)(float)b2[q];
+            c.w3[q] = (double)(float)(-ln2 * bf16_round(w3[q]));
+            c.b1[q] = (double)(float)(-log2e * (double)(float)b1[q]);
+            c.b2[q] = (double)(float)(-log2e * (double)(float)b2[q]);
         }
         c.b3 = (double)(float)b3;
         for (int q = 0; q < 10; ++q) {
             c.shift[q] = (double)(float)shift[q];
-            c.scale[q] = (double)(float)scale[q];
+            c.scale[q] = (double)(float)(-log2e * (double)(float)scale[q]);
         }
     };
     if (o->use_float) set(*(Ctx<float> *)o->impl);

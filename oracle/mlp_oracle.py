@@ -4,7 +4,7 @@ TEST INFRASTRUCTURE ONLY (the checker of the HIP training kernels).
 Parameters are the flat FP32 master vector of the library (W1 [32][10] at 0, b1 at
 320, W2 [32][32] at 352, b2 at 1376, W3 [32] at 1408, b3 at 1440); inputs are the
 sample table rows 0-9 normalised as (raw - shift) * scale; the loss of a sample is
-exp(2y) P + 2 exp(y) Q + R with y = min(MLP(x), -1), P, Q, R from rows 10-12.
+c^2 P + 2 c Q + R with c = clamp(MLP(x), -e^-1, e^-1), P, Q, R from rows 10-12.
 Everything here is float64.  Pinned by central finite differences of `loss` and by
 the torch BF16 conversion (tests/test_closure_oracle.py).
 """
@@ -14,7 +14,9 @@ import numpy as np
 
 OFF = dict(W1=0, b1=320, W2=352, b2=1376, W3=1408, b3=1440)
 NPARAM = 1441
-YMAX = -1.0
+CMAX = float(np.exp(-1.0))       # output clamp: |c| <= e^-1
+CMIN = -CMAX
+LOG2E, LN2 = 1.4426950408889634, 0.6931471805599453
 
 
 def unpack(p):
@@ -49,13 +51,22 @@ def bf16(a):
 
 
 def forward_bf16(p, shift, scale, raw):
-    """Inference as the GPU rounds it: x, W1, W2, W3 and the first hidden layer in
-    BF16, exact arithmetic otherwise."""
+    """Inference as the GPU rounds it: the inputs, W1, W2, W3 and the first hidden layer
+    in BF16, and the hidden layers in base-2 units u = -log2(e) a (scale, b1, b2 times
+    -log2(e) and W3 times -ln 2, each rounded to FP32; a unit's output u / (1 + 2^u) is
+    -SiLU(a) / ln 2); exact arithmetic otherwise."""
     W1, b1, W2, b2, W3, b3 = unpack(np.asarray(p, np.float32).astype(np.float64))
-    x = (np.asarray(raw, np.float32) - np.asarray(shift, np.float32)) * np.asarray(scale, np.float32)
+    f32 = lambda v: np.asarray(v, np.float64).astype(np.float32)
+    scale2 = f32(-LOG2E * np.asarray(scale, np.float32).astype(np.float64))
+    x = (np.asarray(raw, np.float32) - np.asarray(shift, np.float32)) * scale2
     x = bf16(x)
-    h1 = bf16(silu(x @ bf16(W1).T + b1))
-    return silu(h1 @ bf16(W2).T + b2) @ bf16(W3) + b3
+    b1u = f32(-LOG2E * b1).astype(np.float64)
+    b2u = f32(-LOG2E * b2).astype(np.float64)
+    w3u = f32(-LN2 * bf16(W3)).astype(np.float64)
+    u1 = x @ bf16(W1).T + b1u
+    h1 = bf16(u1 / (1.0 + np.exp2(u1)))
+    u2 = h1 @ bf16(W2).T + b2u
+    return (u2 / (1.0 + np.exp2(u2))) @ w3u + b3
 
 
 def normalise(table, shift, scale, rows):
@@ -64,9 +75,8 @@ def normalise(table, shift, scale, rows):
 
 
 def loss(p, x, P, Q, R, inv_norm=1.0):
-    y = np.minimum(forward(p, x), YMAX)
-    e = np.exp(y)
-    return float(np.sum(e * e * P + 2 * e * Q + R) * inv_norm)
+    c = np.clip(forward(p, x), CMIN, CMAX)
+    return float(np.sum(c * c * P + 2 * c * Q + R) * inv_norm)
 
 
 def loss_grad(p, x, P, Q, R, inv_norm=1.0):
@@ -77,10 +87,10 @@ def loss_grad(p, x, P, Q, R, inv_norm=1.0):
     a2 = h1 @ W2.T + b2
     h2 = silu(a2)
     y = h2 @ W3 + b3
-    clamped = y > YMAX
-    e = np.exp(np.minimum(y, YMAX))
-    L = float(np.sum(e * e * P + 2 * e * Q + R) * inv_norm)
-    dy = np.where(clamped, 0.0, (2 * e * e * P + 2 * e * Q) * inv_norm)
+    clamped = (y > CMAX) | (y < CMIN)
+    c = np.clip(y, CMIN, CMAX)
+    L = float(np.sum(c * c * P + 2 * c * Q + R) * inv_norm)
+    dy = np.where(clamped, 0.0, (2 * c * P + 2 * Q) * inv_norm)
     g = np.zeros(NPARAM)
     g[1408:1440] = h2.T @ dy
     g[1440] = dy.sum()

@@ -36,9 +36,12 @@ def _samples(rng, n):
 
 def test_loss_gradient_finite_differences():
     rng = np.random.default_rng(1)
-    p = closure.MLPParams.random(seed=3, y0=-4.0).flat.astype(np.float64)
+    p = closure.MLPParams.random(seed=3, y0=0.02).flat.astype(np.float64)
+    p[closure.OFF_W3:closure.OFF_B3] *= 0.2          # outputs inside the clamp (-e^-1, e^-1)
     x, P, Q, R = _samples(rng, 64)
     L, g = MO.loss_grad(p, x, P, Q, R, 0.5)
+    y = MO.forward(p, x)
+    assert np.all(np.abs(y) < 0.9 * MO.CMAX), "the finite differences must see the unclamped loss"
     assert abs(L - MO.loss(p, x, P, Q, R, 0.5)) < 1e-12 * abs(L)
     h = 1e-6
     idx = np.concatenate([rng.choice(np.arange(0, 320), 12, replace=False), np.arange(320, 324),
@@ -53,10 +56,11 @@ def test_loss_gradient_finite_differences():
 
 def test_loss_gradient_clamped_output_is_flat():
     rng = np.random.default_rng(2)
-    p = closure.MLPParams.constant(0.0).flat.astype(np.float64)   # y = 0 > -1: clamped
     x, P, Q, R = _samples(rng, 8)
-    _, g = MO.loss_grad(p, x, P, Q, R)
-    assert np.all(g == 0.0)
+    for y0 in (0.5, -0.5):          # beyond e^-1 on either side: clamped
+        p = closure.MLPParams.constant(y0).flat.astype(np.float64)
+        _, g = MO.loss_grad(p, x, P, Q, R)
+        assert np.all(g == 0.0)
 
 
 def _small_grid():
@@ -83,7 +87,7 @@ def test_oracle_network_matches_numpy():
 
 
 def test_constant_network_is_smagorinsky():
-    """nu_0 = exp(2 log C_s) Delta^2 |S| + 1e-5 = |S| (C_s Delta)^2 + 1e-5."""
+    """nu_0 = 1e-5 + C_s^2 Delta^2 |S| = |S| (C_s Delta)^2 + 1e-5."""
     prm = atmos.ideal_gas_parameters()
     grid = uniform_brick((2, 2, 2), (0, 0, 0), (300.0, 300.0, 300.0), N=4, periodicity=(True, True, False),
                          boundary=((0, 0), (0, 0), (1, 2)))
@@ -97,9 +101,9 @@ def test_constant_network_is_smagorinsky():
     Q[:, 1:4, :] += np.random.default_rng(6).uniform(-1, 1, Q[:, 1:4, :].shape)
     ref = O.Oracle(cs, grid, aux).tendency(Q)
     ol = O.Oracle(cl, grid, aux)
-    ol.set_mlp(closure.MLPParams.constant(2 * math.log(0.23)))
+    ol.set_mlp(closure.MLPParams.constant(0.23 ** 2))
     got = ol.tendency(Q)
-    # the only difference is y = 2 log C_s held in FP32 (relative 1e-7 in nu_0), and
+    # the only difference is y = C_s^2 held in FP32 (relative 1e-7 in nu_0), and
     # the viscous part is a small fraction of the tendency
     for s in range(5):
         rel = np.linalg.norm(got[:, s] - ref[:, s]) / np.linalg.norm(ref[:, s])
@@ -116,7 +120,7 @@ def test_constant_network_is_smagorinsky():
     # with a kinematic viscosity in the descriptor the closure's floor is that viscosity
     cv = atmos.AtmosConfig(prm, turbulence=abi.LEARNED_CLOSURE, viscosity=0.3, **kw)
     ov = O.Oracle(cv, grid, aux)
-    ov.set_mlp(closure.MLPParams.constant(2 * math.log(0.23)))
+    ov.set_mlp(closure.MLPParams.constant(0.23 ** 2))
     ov.tendency(Q)
     assert np.allclose(ov.gradient_flux()[:, 19], normS * (0.23 * Delta) ** 2 + np.float32(0.3), rtol=3e-7, atol=0)
     # the velocity gradient the network sees: its symmetric part is S

@@ -45,7 +45,7 @@ def test_brick_context_is_bitwise_the_array_context(case):
     for grid in (full, lean):
         dg = DGModel(cfg, grid, aux=aux, precision=precision)
         if cfg.turbulence == abi.LEARNED_CLOSURE:
-            dg.set_closure(closure.MLPParams.constant(2 * math.log(0.21)))
+            dg.set_closure(closure.MLPParams.constant(0.21 ** 2))
         Q = to_device(Q0)
         T = torch.full_like(Q, float("nan"))
         dg(T, Q, None, 0.0)

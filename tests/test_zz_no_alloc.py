@@ -33,7 +33,8 @@ def test_static_loss_gradient_equals_the_workspace_entry_point():
     lib = abi.load_library()
     stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
     rng = np.random.default_rng(11)
-    mlp = closure.MLPParams.random(seed=4, y0=-3.0, shift=rng.normal(0, 0.1, 10), scale=rng.uniform(0.3, 1.0, 10))
+    mlp = closure.MLPParams.random(seed=4, y0=0.02, shift=rng.normal(0, 0.1, 10), scale=rng.uniform(0.3, 1.0, 10))
+    mlp.flat[closure.OFF_W3:closure.OFF_B3] *= 0.1   # outputs inside the clamp: non-zero gradients
     f32 = dict(dtype=torch.float32, device="cuda")
     params = torch.from_numpy(mlp.flat.astype(np.float32)).to(**f32)
     shift = torch.from_numpy(mlp.shift.astype(np.float32)).to(**f32)
@@ -60,7 +61,7 @@ def test_static_loss_gradient_equals_the_workspace_entry_point():
         assert rc == abi.CMDG_OK
         torch.cuda.synchronize()
         a, b = g_ws.cpu().numpy(), g_st.cpu().numpy()
-        assert np.isfinite(b).all()
+        assert np.isfinite(b).all() and np.abs(b[:abi.MLP_NPARAM]).max() > 0
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), "static entry point differs from the workspace one"
         rows = np.arange(batch) if idx is None else idx.cpu().numpy()
         x = MO.normalise(table, mlp.shift, mlp.scale, rows)
