@@ -643,9 +643,13 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
     int rc;
     if ((rc = upload(c, &c->d_D, d->D, (size_t)Nq * Nq))) return rc;
     if ((rc = upload(c, &c->d_aux, d->aux_init, (size_t)Np * (d->naux > 0 ? d->naux : 1) * E))) return rc;
-    if ((rc = upload<double>(c, &c->d_gf, nullptr, (size_t)Np * c->ngf * E))) return rc;
+    // the FP64 gradient flux of FP64 contexts (mixed ones keep theirs in FP32, d_gf32)
+    if (d->precision != CMDG_MIXED && (rc = upload<double>(c, &c->d_gf, nullptr, (size_t)Np * c->ngf * E)))
+        return rc;
     if ((rc = upload<double>(c, &c->d_Qs, nullptr, (size_t)Np * 5 * E))) return rc;
-    if ((rc = check_hip(c, hipMemset(c->d_gf, 0, (size_t)Np * c->ngf * E * sizeof(double)), "hipMemset"))) return rc;
+    if (c->d_gf &&
+        (rc = check_hip(c, hipMemset(c->d_gf, 0, (size_t)Np * c->ngf * E * sizeof(double)), "hipMemset")))
+        return rc;
     c->A.D = c->d_D;
     c->A.nelem = E;
     std::vector<int64_t> crep;  // first element of each static-field class (empty: no classes)
@@ -782,6 +786,8 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
         M.gravity = P.gravity;
         M.rusanov = P.rusanov;
         if ((rc = upload<float>(c, &c->d_gf32, nullptr, (size_t)Np * c->ngf * E))) return rc;
+        if ((rc = check_hip(c, hipMemset(c->d_gf32, 0, (size_t)Np * c->ngf * E * sizeof(float)), "hipMemset")))
+            return rc;
         if (P.a_gPhi >= 0) {
             // grad Phi enters the mixed kernels in FP32 only: keep it rounded, 12 B a node
             // (per static-field class when there are classes)
@@ -1133,24 +1139,28 @@ int cmdg_courant(cmdg_ctx *c, const double *Q, double dt, int32_t kind, int32_t 
     if (!Q || !out) return fail(c, CMDG_EINVAL, "null pointer");
     if (kind < 0 || kind > 2 || direction < 0 || direction > 2) return fail(c, CMDG_EINVAL, "bad Courant kind or direction");
     if (c->aux_coord < 0) return fail(c, CMDG_EINVAL, "the Courant number needs coordinates in aux");
-    if (kind == COURANT_DIFFUSIVE && c->mixed)
-        return fail(c, CMDG_EUNSUPPORTED, "the diffusive Courant number reads the FP64 gradient flux");
     int rc = diag_scratch(c, c->nelem);
     if (rc) return rc;
     double *part = reinterpret_cast<double *>(c->d_part);
-    const double *gf = (kind == COURANT_DIFFUSIVE && c->diffusive) ? c->d_gf : nullptr;
-    const int ngf = gf ? c->ngf : 0;
+    const bool diff = kind == COURANT_DIFFUSIVE && c->diffusive;
+    const double *gf = (diff && !c->mixed) ? c->d_gf : nullptr;
+    const float *gf32 = (diff && c->mixed) ? c->d_gf32 : nullptr;
+    const int ngf = diff ? c->ngf : 0;
     dim3 grid((unsigned)c->nelem);
-#define CMDG_COURANT(NQ, SM)                                                                                     \
-    hipLaunchKernelGGL((k_courant<NQ, SM>), grid, dim3(threads_for<NQ>()), 0, c->stream, c->P, Q, c->d_aux, gf, \
-                       ngf, c->nelem, dt, kind, direction, c->aux_coord, part)
-    if (c->Nq == 5) {
-        if (c->smag) CMDG_COURANT(5, true);
-        else CMDG_COURANT(5, false);
-    } else {
-        if (c->smag) CMDG_COURANT(4, true);
-        else CMDG_COURANT(4, false);
-    }
+#define CMDG_COURANT(NQ, SM, MIX, LEARN)                                                                        \
+    hipLaunchKernelGGL((k_courant<NQ, SM, MIX, LEARN>), grid, dim3(threads_for<NQ>()), 0, c->stream, c->P, c->MP, \
+                       Q, c->d_aux, gf, gf32, ngf, c->nelem, dt, kind, direction, c->aux_coord, part)
+#define CMDG_COURANT_NQ(NQ)                                  \
+    do {                                                     \
+        if (gf32 && c->learned) CMDG_COURANT(NQ, true, true, true);  \
+        else if (gf32 && c->smag) CMDG_COURANT(NQ, true, true, false); \
+        else if (gf32) CMDG_COURANT(NQ, false, true, false);   \
+        else if (c->smag) CMDG_COURANT(NQ, true, false, false); \
+        else CMDG_COURANT(NQ, false, false, false);            \
+    } while (0)
+    if (c->Nq == 5) CMDG_COURANT_NQ(5);
+    else CMDG_COURANT_NQ(4);
+#undef CMDG_COURANT_NQ
 #undef CMDG_COURANT
     hipLaunchKernelGGL(k_max_reduce, dim3(1), dim3(256), 0, c->stream, part, c->nelem, c->d_scalar);
     return diag_result(c, out);

@@ -7,16 +7,21 @@
 // src/Arrays/MPIStateArrays.jl:655-674, double-double accumulation).
 #pragma once
 #include "cmdg_physics.h"
+#include "cmdg_mixed.h"
 
 namespace cmdg {
 
 enum { COURANT_ADVECTIVE = 0, COURANT_NONDIFFUSIVE = 1, COURANT_DIFFUSIVE = 2 };
 enum { DIR_EVERY = 0, DIR_HORIZONTAL = 1, DIR_VERTICAL = 2 };
 
-template <int Nq, bool SMAG>
+// MIX: the gradient flux is the FP32 one of a mixed-precision context (gf32), and the
+// diffusive Courant number takes the viscosity as the mixed kernels form it (LEARN: nu_0
+// of the learned closure, gradient-flux entry 10).
+template <int Nq, bool SMAG, bool MIX = false, bool LEARN = false>
 __global__ void __launch_bounds__(threads_for<Nq>())
-k_courant(KParams P, const double *__restrict__ Q, const double *__restrict__ aux, const double *__restrict__ gf,
-          int ngf, int64_t nelem, double dt, int kind, int direction, int coord, double *__restrict__ partial) {
+k_courant(KParams P, MixParams MP, const double *__restrict__ Q, const double *__restrict__ aux,
+          const double *__restrict__ gf, const float *__restrict__ gf32, int ngf, int64_t nelem, double dt, int kind,
+          int direction, int coord, double *__restrict__ partial) {
     constexpr int Np = Nq * Nq * Nq, NT = threads_for<Nq>();
     __shared__ double sx[3][Np];
     __shared__ double red[NT];
@@ -49,11 +54,20 @@ k_courant(KParams P, const double *__restrict__ Q, const double *__restrict__ au
         const Aux a = load_aux(P, aux, Np, n, e);
         const double kh[3] = {a.gPhi[0] / P.grav, a.gPhi[1] / P.grav, a.gPhi[2] / P.grav};
         if (kind == COURANT_DIFFUSIVE) {
-            double g[11] = {};
-            const double *gb = gf + (int64_t)Np * ngf * e + n;
-            for (int s = 0; s < ngf && s < 11; ++s) g[s] = gb[(int64_t)Np * s];
             double nu[3];
-            viscosity<SMAG>(P, q[0], g, a.gPhi, a.Delta, nu);
+            if (MIX) {
+                float g[11] = {}, nf[3];
+                const float *gb = gf32 + (int64_t)Np * ngf * e + n;
+                for (int s = 0; s < ngf && s < 11; ++s) g[s] = gb[(int64_t)Np * s];
+                const float gp[3] = {(float)a.gPhi[0], (float)a.gPhi[1], (float)a.gPhi[2]};
+                viscosity_f<SMAG, LEARN>(MP, (float)q[0], g, gp, (float)a.Delta, nf);
+                for (int d = 0; d < 3; ++d) nu[d] = (double)nf[d];
+            } else {
+                double g[11] = {};
+                const double *gb = gf + (int64_t)Np * ngf * e + n;
+                for (int s = 0; s < ngf && s < 11; ++s) g[s] = gb[(int64_t)Np * s];
+                viscosity<SMAG>(P, q[0], g, a.gPhi, a.Delta, nu);
+            }
             double nn;
             if (!SMAG) {
                 nn = nu[0];  // a scalar viscosity in every direction

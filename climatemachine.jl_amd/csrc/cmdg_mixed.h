@@ -146,14 +146,15 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 }
 
 // ------------------------------------------------------------------ gradient (mixed)
-// Five waves per SIMD: the Smagorinsky-Lilly variant waits on memory more than it
-// computes (the learned one is VALU-bound), and the register cap this asks for (96)
-// spills nothing at Nq = 5.
+// Five waves per SIMD with a gravitational potential, six without at Nq = 5 (its theta_v
+// work gone; the register caps this asks for, 96 and 80, spill nothing there): the
+// Smagorinsky-Lilly variant waits on memory more than it computes (the learned one is
+// VALU-bound).
 // BUOY: the context has a gravitational potential (FlatOrientation), so Smagorinsky-Lilly's
 // N^2 = grad theta_v . grad Phi / theta_v needs theta_v and its gradient; without one
 // (grad Phi = 0, N^2 = 0 exactly) the kernel evaluates neither.
 template <int Nq, bool SMAG, bool LEARN, bool BUOY>
-__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu((Nq == 5 && !BUOY) ? 6 : 5)))
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
                double *__restrict__ aux, float *__restrict__ gfo,
                float *__restrict__ f2tr, int write_aux, const MlpDev *__restrict__ mlp) {

@@ -277,7 +277,8 @@ int cmdg_interpolate_equispaced(const double *Q, const int32_t *eijk, int64_t ne
  * node spacing of kernel_min_neighbor_distance! (src/Numerics/Mesh/Grids.jl:1228-1334)
  * taken from the aux coordinates.  kind: 0 advective dt|u|/dx, 1 nondiffusive
  * dt(|u| + c)/dx, 2 diffusive dt nu/dx^2 (nu from the gradient flux of the last RHS,
- * FP64 contexts); direction: 0 EveryDirection, 1 HorizontalDirection,
+ * formed as that RHS's kernels form it: FP64, or FP32 in mixed contexts, the learned nu_0
+ * with the learned closure); direction: 0 EveryDirection, 1 HorizontalDirection,
  * 2 VerticalDirection.  Synchronises; *courant is a host double. */
 int cmdg_courant(cmdg_ctx *ctx, const double *Q, double dt, int32_t kind, int32_t direction,
                  double *courant);

@@ -306,6 +306,36 @@ def test_courant_and_weightedsum(name):
     dg.close()
 
 
+@pytest.mark.parametrize("name", ["les_walls", "tgv_smagorinsky", "viscous_periodic", "les_learned_constant"])
+def test_mixed_diffusive_courant(name):
+    """The diffusive Courant number of a mixed-precision context, from its FP32 gradient
+    flux (and, with the learned closure, the learned nu_0), against the oracle's FP64
+    value: the network whose output is C_s^2 is Smagorinsky-Lilly, so its oracle is the
+    Smagorinsky-Lilly one.  Every direction, to the FP32 precision of the viscosity."""
+    import torch
+    from cmdg import closure
+    from cmdg.dg import DGModel, to_device
+    print(f"[mixed] diffusive courant {name}", flush=True)
+    learned = name == "les_learned_constant"
+    cfg, grid, aux, Q0 = _case("les_walls" if learned else name)
+    orc = O.Oracle(cfg, grid, aux)
+    orc.tendency(Q0)
+    cfg_d = atmos.AtmosConfig(cfg.params, orientation=cfg.orientation, ref_profile=cfg.ref_profile,
+                              gravity_source=cfg.gravity_source, turbulence=abi.LEARNED_CLOSURE,
+                              C_smag=cfg.C_smag) if learned else cfg
+    dg = DGModel(cfg_d, grid, aux=aux, precision=abi.MIXED)
+    if learned:
+        dg.set_closure(closure.MLPParams.constant(cfg.C_smag ** 2))
+    Q = to_device(Q0)
+    dg(torch.zeros_like(Q), Q, None, 0.0)
+    for direction, d in (("every", 0), ("horizontal", 1), ("vertical", 2)):
+        got = dg.courant(Q, 0.01, "diffusive", direction)
+        ref = orc.courant(Q0, 0.01, 2, d)
+        print(f"  {direction}: {got!r} against {ref!r}", flush=True)
+        assert abs(got - ref) <= 1e-5 * abs(ref), (direction, got, ref)
+    dg.close()
+
+
 def test_config1_vortex_100_steps():
     """The reference's CPU-runnable case: isentropic vortex, 10 m periodic box of
     8 x 8 x 2 elements, N = 4, rho = 1, u = (1, 1, 0), T = 300 K, vortex strength 5,
