@@ -41,13 +41,15 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16_mfma": 2500.0}  # MI355X dense peaks
 # Algorithmic HBM bytes per node of one launch of the chosen form's kernels:
 # LSRK-fused tendency: Q read + dQ read + dQ write + Q write (4 x 5 x 8 = 160), FP32
-# gradient flux read (11 x 4 = 44: gradient of h, S, N^2, learned nu_0), FP32 geometry
-# xi (9) + MI (1) (40), aux rho_ref, p_ref, Phi (FP64, 24; absent on the Taylor-Green
-# vortex); gradient kernel: Q read 40, aux 24 (LES), gradient flux write 44.
-# Whole stage: Q, dQ 160, geometry 40, aux 24 (LES), gradient flux written and read 88.
+# gradient flux in stress form read (the stress factor and the enthalpy diffusion: 9 x 4 =
+# 36 with an isotropic viscosity, the Taylor-Green vortex; 12 x 4 = 48 with the buoyancy
+# factor, the LES bubble), FP32 geometry xi (9) + MI (1) (40), aux rho_ref, p_ref, Phi
+# (FP64, 24; absent on the Taylor-Green vortex); gradient kernel: Q read 40, aux 24 (LES),
+# stress form written (10 x 4 = 40, or 15 x 4 = 60: with the viscosity planes).
+# Whole stage: Q, dQ 160, geometry 40, aux 24 (LES), stress form written and read.
 BYTES = {
-    "les25": {"tendency": 160 + 44 + 40 + 24, "gradient": 40 + 24 + 44, "stage": 160 + 40 + 24 + 88},
-    "tgv96": {"tendency": 160 + 44 + 40, "gradient": 40 + 44, "stage": 160 + 40 + 88},
+    "les25": {"tendency": 160 + 48 + 40 + 24, "gradient": 40 + 24 + 60, "stage": 160 + 40 + 24 + 60 + 48},
+    "tgv96": {"tendency": 160 + 36 + 40, "gradient": 40 + 40, "stage": 160 + 40 + 40 + 36},
 }
 WEIGHTS = os.path.join(ROOT, "climatemachine.jl_amd", "cmdg", "data", "tgv_closure.npz")
 TRAIN_META = os.path.join(ROOT, "climatemachine.jl_amd", "cmdg", "data", "tgv_closure_train.json")

@@ -65,6 +65,22 @@ def to_host(t: torch.Tensor) -> np.ndarray:
     return np.asfortranarray(t.detach().cpu().numpy().T)
 
 
+def stress_form_layout(ngradflux: int) -> dict:
+    """Planes of the mixed-precision gradient flux (stress form) for its plane count:
+    ``tau`` the viscous stress factor -2 nu_d S_d. (+ 2 nu / 3 tr S on the diagonal
+    with divergence): 6 symmetric planes xx xy xz yy yz zz when the viscosity is
+    isotropic (``iso``, 10 planes in all), else 9 row-major (15 planes); ``e`` the
+    three (nu_d / Pr_t) dh_tot/dx_d; ``nu`` the viscosity (1 plane, or nu_d).  Row d of
+    the second-order flux is (0, rho tau_d., tau_d . rho u - rho e_d)."""
+    if ngradflux not in (10, 15):
+        raise CmdgError(f"not a stress-form plane count: {ngradflux}")
+    iso = ngradflux == 10
+    ntau = 6 if iso else 9
+    sym = ((0, 1, 2), (1, 3, 4), (2, 4, 5))
+    tau = [[sym[d][c] if iso else 3 * d + c for c in range(3)] for d in range(3)]
+    return {"iso": iso, "tau": tau, "e": [ntau, ntau + 1, ntau + 2], "nu": list(range(ntau + 3, ngradflux))}
+
+
 def _ptr(t: torch.Tensor) -> int:
     if t.dtype != torch.float64 or not t.is_contiguous() or not t.is_cuda:
         raise CmdgError("state arrays must be contiguous float64 CUDA tensors")
@@ -128,7 +144,8 @@ class DGModel:
 
     @property
     def state_gradient_flux_f32(self) -> torch.Tensor:
-        """Device view (nelem, ngradflux, Np) of the mixed-precision FP32 gradient flux."""
+        """Device view (nelem, ngradflux, Np) of the mixed-precision FP32 gradient flux, in
+        stress form (``stress_form_layout`` names its planes)."""
         p = C.c_void_p()
         n = C.c_int32()
         self._check(self.lib.cmdg_gradient_flux_f32(self._ctx, C.byref(p), C.byref(n)), "cmdg_gradient_flux_f32")

@@ -97,7 +97,11 @@ struct cmdg_ctx {
     // mixed precision (FP32 fluxes and gradient flux)
     bool mixed = false;
     MixParams MP{};
-    float *d_gf32 = nullptr, *d_f2tr32 = nullptr, *d_gphi32 = nullptr;
+    // the FP32 gradient flux in stress form: iso32 its isotropic layout (no buoyancy
+    // correction of the viscosity), ngf32 its planes
+    float *d_gf32 = nullptr, *d_gphi32 = nullptr;
+    bool iso32 = true;
+    int ngf32 = 0;
     // learned closure (mixed precision only)
     bool learned = false, mlp_ready = false;
     MlpDev *d_mlp = nullptr;
@@ -168,7 +172,7 @@ int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_
     const bool buoy = c->P.a_gPhi >= 0;
 #define CMDG_GRAD_MIX(SMAG, LEARN, BUOY)                                                                       \
     hipLaunchKernelGGL((k_gradient_mix<Nq, SMAG, LEARN, BUOY>), grid, block, 0, c->stream, c->P, c->MP, c->AA, ES, \
-                       Q, c->d_aux, c->d_gf32, c->d_f2tr32, write_aux, c->d_mlp)
+                       Q, c->d_aux, c->d_gf32, write_aux, c->d_mlp)
 #define CMDG_GRAD_AFF(SMAG, BUOY)                                                                               \
     hipLaunchKernelGGL((k_gradient_aff<Nq, SMAG, BUOY>), grid, block, 0, c->stream, c->P, c->AA, ES, Q, c->d_aux, \
                        c->d_gf, c->d_f2tr, write_aux)
@@ -202,14 +206,15 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alph
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)xcd_grid(c->affine ? ES.count : c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_TENDENCY);
-#define CMDG_TEND_MIX(DIFF, SMAG, LEARN, MODE)                                                                   \
-    hipLaunchKernelGGL((k_tendency_mix<Nq, DIFF, SMAG, LEARN, MODE>), grid, block, 0, c->stream, c->P, c->MP, c->AA, \
-                       ES, Q, c->d_gf32, c->d_f2tr32, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux)
+#define CMDG_TEND_MIX(DIFF, ISO, MODE)                                                                           \
+    hipLaunchKernelGGL((k_tendency_mix<Nq, DIFF, ISO, MODE>), grid, block, 0, c->stream, c->P, c->MP, c->AA, ES, Q, \
+                       c->d_gf32, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux)
 #define CMDG_TEND(DIFF, SMAG, MODE)                                                                               \
     do {                                                                                                          \
-        if (c->mixed)                                                                                             \
-            CMDG_TEND_MIX(DIFF, SMAG, false, MODE);                                                               \
-        else if (c->affine)                                                                                       \
+        if (c->mixed) {                                                                                           \
+            if (!DIFF || c->iso32) CMDG_TEND_MIX(DIFF, true, MODE);                                               \
+            else CMDG_TEND_MIX(DIFF, false, MODE);                                                                \
+        } else if (c->affine)                                                                                     \
             hipLaunchKernelGGL((k_tendency_aff<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->AA,    \
                                ES, Q, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, rka, bdt,      \
                                write_aux);                                                                        \
@@ -217,10 +222,7 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alph
             hipLaunchKernelGGL((k_tendency<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->A, Q,      \
                                c->d_gf, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux);                 \
     } while (0)
-    if (c->learned) {
-        if (mode == 0) CMDG_TEND_MIX(true, true, true, 0);
-        else CMDG_TEND_MIX(true, true, true, 1);
-    } else if (mode == 0) {
+    if (mode == 0) {
         if (!c->diffusive) CMDG_TEND(false, false, 0);
         else if (c->smag) CMDG_TEND(true, true, 0);
         else CMDG_TEND(true, false, 0);
@@ -728,11 +730,12 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
             }
             // tiled traversal (CMDG_ELEMENT_TILE=T sets the tile edge, 0 keeps the storage
             // order; CMDG_TILED_PASSES = tendency, gradient, both or none chooses the
-            // passes that walk the tiles).  Measured on one box: tiles for the tendency
-            // pass alone are the fastest of the four for both precisions -- mixed + learned
-            // closure +1.8 % at 25^3 against storage order and +2.3 % at 96^3 against tiles
-            // for both passes, FP64 +1.8 % at 96^3 against storage order (within noise at
-            // 25^3).  The gradient pass loses in tiles about what its gathers gain.
+            // passes that walk the tiles).  Default: both passes for mixed contexts, the
+            // tendency pass alone for FP64 ones.  Measured at 96^3 on one box: mixed +
+            // learned closure 55.0 (tendency pass) -> 56.0 GDOF/s (both; tile edge 6 / 12 /
+            // 16: 55.3 / 56.2 / 55.8), mixed + Smagorinsky-Lilly 59.6 -> 60.3; FP64 40.4
+            // (tendency pass) against 38.6 (both): the FP64 gradient pass, which writes its
+            // second-order face trace, loses in tiles more than its gathers gain.
             const char *tile_env = std::getenv("CMDG_ELEMENT_TILE");
             const int T = tile_env ? std::atoi(tile_env) : 8;
             std::vector<int32_t> order;
@@ -740,7 +743,7 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
             if (brick ? tiled_order_lattice(lat, nax, E, T, order) : tiled_order(d, Np, T, order)) {
                 if ((rc = upload(c, &c->d_order, order.data(), order.size()))) return rc;
                 const char *penv = std::getenv("CMDG_TILED_PASSES");
-                const std::string passes = penv ? penv : "tendency";
+                const std::string passes = penv ? penv : (d->precision == CMDG_MIXED ? "both" : "tendency");
                 if (passes == "both" || passes == "gradient") c->ord_grad = c->d_order;
                 if (passes == "both" || passes == "tendency") c->ord_tend = c->d_order;
             }
@@ -785,8 +788,10 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
         M.ref_src = P.ref_src;
         M.gravity = P.gravity;
         M.rusanov = P.rusanov;
-        if ((rc = upload<float>(c, &c->d_gf32, nullptr, (size_t)Np * c->ngf * E))) return rc;
-        if ((rc = check_hip(c, hipMemset(c->d_gf32, 0, (size_t)Np * c->ngf * E * sizeof(float)), "hipMemset")))
+        c->iso32 = !(c->smag && P.a_gPhi >= 0);
+        c->ngf32 = c->iso32 ? StressForm<true>::NGF : StressForm<false>::NGF;
+        if ((rc = upload<float>(c, &c->d_gf32, nullptr, (size_t)Np * c->ngf32 * E))) return rc;
+        if ((rc = check_hip(c, hipMemset(c->d_gf32, 0, (size_t)Np * c->ngf32 * E * sizeof(float)), "hipMemset")))
             return rc;
         if (P.a_gPhi >= 0) {
             // grad Phi enters the mixed kernels in FP32 only: keep it rounded, 12 B a node
@@ -803,7 +808,6 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
             if ((rc = upload(c, &c->d_gphi32, g.data(), g.size()))) return rc;
             c->AA.gphi32 = c->d_gphi32;
         }
-        if ((rc = upload<float>(c, &c->d_f2tr32, nullptr, (size_t)E * 6 * TR_F2 * Nfp))) return rc;
         if (c->learned && (rc = upload<MlpDev>(c, &c->d_mlp, nullptr, 1))) return rc;
     }
     return CMDG_OK;
@@ -838,7 +842,6 @@ int cmdg_destroy(cmdg_ctx *c) {
     (void)hipFree(c->d_order);
     (void)hipFree(c->d_gf32);
     (void)hipFree(c->d_gphi32);
-    (void)hipFree(c->d_f2tr32);
     (void)hipFree(c->d_mlp);
     (void)hipFree(c->d_part);
     (void)hipFree(c->d_scalar);
@@ -933,7 +936,7 @@ int cmdg_gradient_flux_f32(cmdg_ctx *c, float **gf, int32_t *ngf) {
     if (!c || !gf) return CMDG_EINVAL;
     if (!c->mixed) return fail(c, CMDG_EUNSUPPORTED, "the FP64 path stores its gradient flux in FP64");
     *gf = c->d_gf32;
-    if (ngf) *ngf = c->ngf;
+    if (ngf) *ngf = c->ngf32;
     return CMDG_OK;
 }
 
@@ -1145,18 +1148,18 @@ int cmdg_courant(cmdg_ctx *c, const double *Q, double dt, int32_t kind, int32_t 
     const bool diff = kind == COURANT_DIFFUSIVE && c->diffusive;
     const double *gf = (diff && !c->mixed) ? c->d_gf : nullptr;
     const float *gf32 = (diff && c->mixed) ? c->d_gf32 : nullptr;
-    const int ngf = diff ? c->ngf : 0;
+    const int ngf = diff ? (c->mixed ? c->ngf32 : c->ngf) : 0;
     dim3 grid((unsigned)c->nelem);
-#define CMDG_COURANT(NQ, SM, MIX, LEARN)                                                                        \
-    hipLaunchKernelGGL((k_courant<NQ, SM, MIX, LEARN>), grid, dim3(threads_for<NQ>()), 0, c->stream, c->P, c->MP, \
+#define CMDG_COURANT(NQ, SM, MIX, ISO)                                                                          \
+    hipLaunchKernelGGL((k_courant<NQ, SM, MIX, ISO>), grid, dim3(threads_for<NQ>()), 0, c->stream, c->P, c->MP, \
                        Q, c->d_aux, gf, gf32, ngf, c->nelem, dt, kind, direction, c->aux_coord, part)
-#define CMDG_COURANT_NQ(NQ)                                  \
-    do {                                                     \
-        if (gf32 && c->learned) CMDG_COURANT(NQ, true, true, true);  \
-        else if (gf32 && c->smag) CMDG_COURANT(NQ, true, true, false); \
-        else if (gf32) CMDG_COURANT(NQ, false, true, false);   \
-        else if (c->smag) CMDG_COURANT(NQ, true, false, false); \
-        else CMDG_COURANT(NQ, false, false, false);            \
+#define CMDG_COURANT_NQ(NQ)                                             \
+    do {                                                                \
+        if (gf32 && c->smag && c->iso32) CMDG_COURANT(NQ, true, true, true); \
+        else if (gf32 && c->smag) CMDG_COURANT(NQ, true, true, false);  \
+        else if (gf32) CMDG_COURANT(NQ, false, true, true);             \
+        else if (c->smag) CMDG_COURANT(NQ, true, false, true);          \
+        else CMDG_COURANT(NQ, false, false, true);                      \
     } while (0)
     if (c->Nq == 5) CMDG_COURANT_NQ(5);
     else CMDG_COURANT_NQ(4);

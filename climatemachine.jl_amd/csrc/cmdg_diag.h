@@ -14,10 +14,10 @@ namespace cmdg {
 enum { COURANT_ADVECTIVE = 0, COURANT_NONDIFFUSIVE = 1, COURANT_DIFFUSIVE = 2 };
 enum { DIR_EVERY = 0, DIR_HORIZONTAL = 1, DIR_VERTICAL = 2 };
 
-// MIX: the gradient flux is the FP32 one of a mixed-precision context (gf32), and the
-// diffusive Courant number takes the viscosity as the mixed kernels form it (LEARN: nu_0
-// of the learned closure, gradient-flux entry 10).
-template <int Nq, bool SMAG, bool MIX = false, bool LEARN = false>
+// MIX: the gradient flux is the FP32 stress form of a mixed-precision context (gf32, ISO
+// its isotropic layout), and the diffusive Courant number takes the viscosity the mixed
+// gradient kernel formed (with the learned closure, from the learned nu_0).
+template <int Nq, bool SMAG, bool MIX = false, bool ISO = true>
 __global__ void __launch_bounds__(threads_for<Nq>())
 k_courant(KParams P, MixParams MP, const double *__restrict__ Q, const double *__restrict__ aux,
           const double *__restrict__ gf, const float *__restrict__ gf32, int ngf, int64_t nelem, double dt, int kind,
@@ -55,12 +55,16 @@ k_courant(KParams P, MixParams MP, const double *__restrict__ Q, const double *_
         const double kh[3] = {a.gPhi[0] / P.grav, a.gPhi[1] / P.grav, a.gPhi[2] / P.grav};
         if (kind == COURANT_DIFFUSIVE) {
             double nu[3];
-            if (MIX) {
-                float g[11] = {}, nf[3];
-                const float *gb = gf32 + (int64_t)Np * ngf * e + n;
-                for (int s = 0; s < ngf && s < 11; ++s) g[s] = gb[(int64_t)Np * s];
-                const float gp[3] = {(float)a.gPhi[0], (float)a.gPhi[1], (float)a.gPhi[2]};
-                viscosity_f<SMAG, LEARN>(MP, (float)q[0], g, gp, (float)a.Delta, nf);
+            if (MIX && SMAG) {
+                // the viscosity the gradient kernel formed from the last RHS's gradient
+                using L = StressForm<ISO>;
+                const float *gb = gf32 + (int64_t)Np * L::NGF * e + n;
+                for (int d = 0; d < 3; ++d) nu[d] = (double)gb[(int64_t)Np * (ISO ? L::NU : L::NU + d)];
+            } else if (MIX) {
+                // a constant viscosity (mu / rho: of this state)
+                float nf[3];
+                const float g0[1] = {0.0f}, gp[3] = {0.0f, 0.0f, 0.0f};
+                viscosity_f<false>(MP, (float)q[0], g0, gp, 0.0f, nf);
                 for (int d = 0; d < 3; ++d) nu[d] = (double)nf[d];
             } else {
                 double g[11] = {};

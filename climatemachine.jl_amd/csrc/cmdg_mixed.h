@@ -7,9 +7,10 @@
 // rho - rho_ref (gravity source) are formed in FP64 before rounding.  Without a
 // subtracted reference the momentum flux carries p minus an element constant.  Fluxes,
 // the gradient argument, tensor-product derivatives, face fluxes, the
-// Smagorinsky-Lilly closure and the gradient flux are FP32; the gradient flux and
-// the second-order-flux trace are stored in FP32.  The Rusanov jump q- - q+ is
-// formed in FP64.  Metric terms are computed once in FP64 and rounded on use.
+// Smagorinsky-Lilly closure and the gradient flux are FP32; the gradient flux is
+// stored in FP32 in stress form (the closure folded in: see StressForm).  The Rusanov
+// jump q- - q+ is formed in FP64.  Metric terms are computed once in FP64 and rounded
+// on use.
 #pragma once
 #include "cmdg_affine.h"
 #include "cmdg_closure.h"
@@ -35,10 +36,8 @@ __device__ __forceinline__ void flux1_row_f(const float q[5], const float u[3], 
     F[4] = ud * (q[4] + p);
 }
 
-// strain_norm_f, viscosity_f and flux2_row_f round every operation (no contraction):
-// the gradient kernel evaluates the second-order flux rows of the face nodes for the
-// neighbours' trace and the tendency kernel evaluates the same rows for its own side,
-// and both must get the same bits for the face fluxes to stay conservative.
+// strain_norm_f and viscosity_f round every operation (no contraction): the viscosity
+// of a node does not depend on where the compiler inlines them.
 __device__ __forceinline__ float strain_norm_f(const float *gf) {
 #pragma clang fp contract(off)
     const float *S = gf + 3;
@@ -71,29 +70,72 @@ __device__ __forceinline__ void viscosity_f(const MixParams &P, float rho, const
     }
 }
 
-template <bool SMAG>
-__device__ __forceinline__ void flux2_row_f(const MixParams &P, const float q[5], const float *gf, const float nu[3],
-                                            int d, float F[5]) {
-#pragma clang fp contract(off)
-    const float *S = gf + 3;
-    const float S0 = pick3(d, S[0], S[1], S[2]);
-    const float S1 = pick3(d, S[1], S[3], S[4]);
-    const float S2 = pick3(d, S[2], S[4], S[5]);
-    const float nud = pick3(d, nu[0], nu[1], nu[2]);
-    const float m2 = -2.0f * nud;
-    float t0 = m2 * S0, t1 = m2 * S1, t2 = m2 * S2;
-    if (!SMAG && P.with_div) {
-        const float dv = (2.0f * nud / 3.0f) * (S[0] + S[3] + S[5]);
-        if (d == 0) t0 += dv;
-        if (d == 1) t1 += dv;
-        if (d == 2) t2 += dv;
+// Mixed-precision gradient flux in stress form: what the second-order flux row needs,
+// formed once per node by the gradient kernel (the kinematic stress of
+// TurbulenceClosures.jl:378-404, 472-499 and the enthalpy diffusivity times the
+// enthalpy gradient, tendencies_momentum.jl:37-42, tendencies_energy.jl:27-56).
+// Planes (Np, NGF, nelem):
+//   tau  row d = -2 nu_d S_d. (plus (2 nu / 3) tr S on the diagonal for a constant
+//        viscosity with divergence).  An isotropic viscosity (no buoyancy correction:
+//        constant viscosity, or Smagorinsky-Lilly without a gravitational potential)
+//        makes it symmetric: 6 planes xx xy xz yy yz zz; otherwise 9 planes, row-major;
+//   e    e_d = (nu_d / Pr_t) dh_tot/dx_d, 3 planes;
+//   nu   the viscosity: 1 plane (isotropic) or 3 (nu_d); only diagnostics read it.
+// Row d of the second-order flux is then (0, tau_d. rho, tau_d . rho u - e_d rho): the
+// tendency reads the first NREAD planes, its own for the volume and face rows and the
+// face neighbour's (4 values at the shared node) for the other side of a face.
+template <bool ISO> struct StressForm {
+    static constexpr int NTAU = ISO ? 6 : 9, E = NTAU, NU = NTAU + 3, NREAD = NTAU + 3;
+    static constexpr int NGF = NTAU + 3 + (ISO ? 1 : 3);
+    // plane of tau[d][c]
+    static __device__ __forceinline__ int tau(int d, int c) {
+        if (!ISO) return 3 * d + c;
+        const int a = d < c ? d : c, b = d < c ? c : d;
+        return a == 0 ? b : (a == 1 ? 2 + b : 5);
     }
+};
+
+// Stress form of one node from its gradient flux gf = [grad h_tot (3), S (6), N2, ...]
+// and viscosity nu[3]; every operation rounded (no contraction), in the order the row
+// below and the face neighbour's row use them.
+template <bool SMAG, bool ISO>
+__device__ __forceinline__ void stress_form_f(const MixParams &P, const float *gf, const float nu[3], float *out) {
+#pragma clang fp contract(off)
+    using L = StressForm<ISO>;
+    const float *S = gf + 3;
+    const float trS = S[0] + S[3] + S[5];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float nud = ISO ? nu[0] : nu[d];  // isotropic: the three are equal
+        const float m2 = -2.0f * nud;
+        const float dv = (2.0f * nud / 3.0f) * trS;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (ISO && c < d) continue;
+            const int a = d < c ? d : c, b = d < c ? c : d;
+            float t = m2 * S[a == 0 ? b : (a == 1 ? 2 + b : 5)];
+            if (!SMAG && P.with_div && c == d) t += dv;
+            out[L::tau(d, c)] = t;
+        }
+        out[L::E + d] = (nud * P.inv_Pr) * gf[d];
+    }
+    if (ISO) {
+        out[L::NU] = nu[0];
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[L::NU + d] = nu[d];
+    }
+}
+
+// Row d of the second-order flux from that row of the stress form (t = tau_d., e = e_d)
+// and the FP32 state.
+__device__ __forceinline__ void flux2_row_stress(const float t[3], float e, const float q[5], float F[5]) {
+#pragma clang fp contract(off)
     F[0] = 0.0f;
-    F[1] = t0 * q[0];
-    F[2] = t1 * q[0];
-    F[3] = t2 * q[0];
-    const float gd = pick3(d, gf[0], gf[1], gf[2]);
-    F[4] = t0 * q[1] + t1 * q[2] + t2 * q[3] - (nud * P.inv_Pr) * gd * q[0];
+    F[1] = t[0] * q[0];
+    F[2] = t[1] * q[0];
+    F[3] = t[2] * q[0];
+    F[4] = t[0] * q[1] + t[1] * q[2] + t[2] * q[3] - e * q[0];
 }
 
 // gradient flux of the 3 x 5 gradient (energy.jl:50-59, TurbulenceClosures.jl:455-470)
@@ -156,12 +198,13 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 template <int Nq, bool SMAG, bool LEARN, bool BUOY>
 __global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu((Nq == 5 && !BUOY) ? 6 : 5)))
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
-               double *__restrict__ aux, float *__restrict__ gfo,
-               float *__restrict__ f2tr, int write_aux, const MlpDev *__restrict__ mlp) {
+               double *__restrict__ aux, float *__restrict__ gfo, int write_aux, const MlpDev *__restrict__ mlp) {
 #pragma clang fp contract(fast)
     static_assert(!LEARN || SMAG, "the learned closure keeps the Smagorinsky-Lilly structure");
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
-    constexpr int NG = SMAG ? 5 : 4, NGF = LEARN ? 11 : (SMAG ? 10 : 9);
+    constexpr int NG = SMAG ? 5 : 4;
+    constexpr bool ISO = !(SMAG && BUOY);
+    using L = StressForm<ISO>;
     constexpr int NT = threads_for<Nq>();
     __shared__ float sD[Nq * Nq];
     __shared__ float sG[NG][Np];
@@ -287,27 +330,14 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         gf[10] = mlp_nu0(y, (float)ge[10], strain_norm_f(gf), MP.visc > 0.0f ? MP.visc : 1e-5f);
     }
     if (act) {
-        float *go = gfo + (int64_t)Np * NGF * e + n;
+        // the stress form of the node's second-order flux: the tendency forms its rows,
+        // and the face neighbours' rows, from it
+        float nu[3], sf[L::NGF];
+        viscosity_f<SMAG, LEARN>(MP, (float)q[0], gf, gPhi, (float)ge[10], nu);
+        stress_form_f<SMAG, ISO>(MP, gf, nu, sf);
+        float *go = gfo + (int64_t)Np * L::NGF * e + n;
 #pragma unroll
-        for (int s = 0; s < NGF; ++s) go[Np * s] = gf[s];
-        // second-order flux rows of the face nodes, written straight to the face-major
-        // trace: the 25 lanes of one face fill its 25 contiguous slots per row entry
-        if (i == 0 || i == Nq - 1 || j == 0 || j == Nq - 1 || k == 0 || k == Nq - 1) {
-            float nu[3], qf[5];
-#pragma unroll
-            for (int s = 0; s < 5; ++s) qf[s] = (float)q[s];
-            viscosity_f<SMAG, LEARN>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                int fn;
-                const int f = node_face_dir<Nq>(i, j, k, d, fn);
-                if (f < 0) continue;
-                float F2[5];
-                flux2_row_f<SMAG>(MP, qf, gf, nu, d, F2);
-#pragma unroll
-                for (int v = 0; v < TR_F2; ++v) f2tr[tri<Nfp>(e, f, v, TR_F2, fn)] = F2[1 + v];
-            }
-        }
+        for (int s = 0; s < L::NGF; ++s) go[Np * s] = sf[s];
     }
 }
 
@@ -317,13 +347,15 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 // along the face's axis, own[4..7]: the second-order row, own[8]: |u_d| + c; the row's
 // mass entry is the FP32 rounding of rho u_d, taken here from the staged state); the
 // neighbour's side is
-// evaluated here from its FP64 volume state (thermodynamics, p - p_ref in FP64), and
-// the jump q- - q+ is formed in FP64 from the own state, staged in LDS with Phi and
+// evaluated here from its FP64 volume state (thermodynamics, p - p_ref in FP64) and, for
+// the second-order row, from its stress form at the shared node (4 of its planes) and
+// its FP32-rounded state: the bits the neighbour's own volume phase forms for that row.
+// The jump q- - q+ is formed in FP64 from the own state, staged in LDS with Phi and
 // p_ref (sQ rows 0-4, 5, 6).
-template <int Nq, bool DIFF>
+template <int Nq, bool DIFF, bool ISO>
 __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams &MP, const AffArrays &A, int f, int fn,
                                               int32_t ne, const double (*sQ)[Nq * Nq * Nq],
-                                              const double *__restrict__ Q, const float *__restrict__ f2tr,
+                                              const double *__restrict__ Q, const float *__restrict__ gfi,
                                               const float own[9], double p_e, float flux[5]) {
 #pragma clang fp contract(fast)
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
@@ -332,12 +364,21 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
     const bool wall = ne < 0;
     const int fo = f ^ 1;
     double q[5], qp[5], Php, refpp;
+    float t2p[3] = {0.0f, 0.0f, 0.0f}, e2p = 0.0f;  // the neighbour's stress-form row
 #pragma unroll
     for (int s = 0; s < 5; ++s) q[s] = sQ[s][vn];
     if (!wall) {
+        // the neighbour's state and stress-form row at the shared node, loaded together
         const double *qb = Q + (int64_t)Np * 5 * ne + face_node<Nq>(fo, fn);
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = qb[Np * s];
+        if (DIFF) {
+            using L = StressForm<ISO>;
+            const float *gn = gfi + (int64_t)Np * L::NGF * ne + face_node<Nq>(fo, fn);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) t2p[c] = gn[Np * L::tau(d, c)];
+            e2p = gn[Np * (L::E + d)];
+        }
         if (A.aux_cont) {
             Php = sQ[5][vn];
             refpp = sQ[6][vn];
@@ -376,25 +417,30 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
         for (int s = 0; s < 5; ++s) flux[s] += mw * jump[s];
     }
     if (DIFF && !wall) {
+        float F2p[5];
+        flux2_row_stress(t2p, e2p, qpf, F2p);
 #pragma unroll
-        for (int v = 0; v < TR_F2; ++v) flux[1 + v] += (own[4 + v] + f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)]) * h;
+        for (int v = 0; v < 4; ++v) flux[1 + v] += (own[4 + v] + F2p[1 + v]) * h;
     }
 }
 
 // MODE 0: tend = alpha*RHS + beta*tend.   MODE 1: LSRK stage (tend is dQ):
 //   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka.  Qout is another buffer than Q:
 //   blocks read their neighbours' volume state.
+// DIFF: second-order fluxes from the gradient flux gfi in stress form (ISO: its
+// isotropic layout); the closure (constant viscosity, Smagorinsky-Lilly or the learned
+// nu_0) is already folded into it.
 // Six waves per SIMD at Nq = 5: the register cap this asks for (80) spills nothing there
 // (at Nq = 4 it would, so that order keeps the compiler's choice).
-template <int Nq, bool DIFF, bool SMAG, bool LEARN, int MODE>
+template <int Nq, bool DIFF, bool ISO, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(Nq == 5 ? 6 : 1)))
 k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
-               const float *__restrict__ gfi, const float *__restrict__ f2tr, double *__restrict__ aux,
-               double *__restrict__ tend, double alpha, double beta, double *__restrict__ Qout, double rka,
-               double bdt, int write_aux) {
+               const float *__restrict__ gfi, double *__restrict__ aux, double *__restrict__ tend, double alpha,
+               double beta, double *__restrict__ Qout, double rka, double bdt, int write_aux) {
 
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
-    constexpr int NGF = LEARN ? 11 : (SMAG ? 10 : 9), NT = threads_for<Nq>();
+    constexpr int NT = threads_for<Nq>();
+    using L = StressForm<ISO>;
     __shared__ float sD[Nq * Nq];
     __shared__ float sF[3][5][Np];
     // per face node: the own side's flux rows and wave speed from the volume phase
@@ -425,7 +471,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         const double Phe = P.a_Phi >= 0 ? A.sa[(int64_t)Np * P.naux * ce + (int64_t)Np * P.a_Phi] : -0.0;
         p_e = thermo_d(P, qe, Phe).p;
     }
-    float qf[5], uf[3], gf[11], nu[3], gPhi[3] = {0.0f, 0.0f, 0.0f}, pf = 0.0f, pmf = 0.0f, cm = 0.0f, drho = 0.0f;
+    float qf[5], uf[3], sf[L::NREAD], gPhi[3] = {0.0f, 0.0f, 0.0f}, pf = 0.0f, pmf = 0.0f, cm = 0.0f, drho = 0.0f;
     float MI = 0.0f;
     if (act) {
         const double *qb = Q + (int64_t)Np * 5 * e + n;
@@ -458,10 +504,9 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         uf[1] = qf[2] * ri;
         uf[2] = qf[3] * ri;
         if (DIFF) {
-            const float *gb = gfi + (int64_t)Np * NGF * e + n;
+            const float *gb = gfi + (int64_t)Np * L::NGF * e + n;
 #pragma unroll
-            for (int s = 0; s < NGF; ++s) gf[s] = gb[Np * s];
-            viscosity_f<SMAG, LEARN>(MP, qf[0], gf, gPhi, (float)ge[10], nu);
+            for (int s = 0; s < L::NREAD; ++s) sf[s] = gb[Np * s];
         }
         const double Md = ge[3] * A.W[n];
         MI = (float)(1.0 / Md);
@@ -478,7 +523,8 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
                 sFc[f * Nfp + fn][8] = fabsf(uf[ax]) + cm;
             }
             if (DIFF) {
-                flux2_row_f<SMAG>(MP, qf, gf, nu, ax, F2);
+                const float t[3] = {sf[L::tau(ax, 0)], sf[L::tau(ax, 1)], sf[L::tau(ax, 2)]};
+                flux2_row_stress(t, sf[L::E + ax], qf, F2);
                 if (f >= 0)
 #pragma unroll
                     for (int v = 0; v < TR_F2; ++v) sFc[f * Nfp + fn][4 + v] = F2[1 + v];
@@ -527,10 +573,16 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         float own[9], fl5[5];
 #pragma unroll
         for (int c = 0; c < 9; ++c) own[c] = sFc[t][c];
-        face_flux_mix<Nq, DIFF>(P, MP, A, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, f2tr, own, p_e, fl5);
+        face_flux_mix<Nq, DIFF, ISO>(P, MP, A, t / Nfp, t % Nfp, nb[t / Nfp], sQ, Q, gfi, own, p_e, fl5);
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFc[t][s] = fl5[s];
     }
+    // the stage's register dQ, loaded before the barrier so that the wait overlaps it
+    double dq[5] = {};
+    double *tb = tend + (int64_t)Np * 5 * e + n;
+    if (MODE == 1 && act)
+#pragma unroll
+        for (int s = 0; s < 5; ++s) dq[s] = tb[Np * s];
     __syncthreads();
 
     if (act) {
@@ -543,17 +595,18 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 #pragma unroll
             for (int s = 0; s < 5; ++s) T[s] -= w * sFc[f * Nfp + fn][s];
         }
-        double *tb = tend + (int64_t)Np * 5 * e + n;
         if (MODE == 0) {
 #pragma unroll
             for (int s = 0; s < 5; ++s)
                 tb[Np * s] = (beta != 0.0) ? alpha * (double)T[s] + beta * tb[Np * s] : alpha * (double)T[s];
         } else {
+            // the stage's state from its LDS copy (sQ is intact after the face tasks): no
+            // register holds it across them
             double *qo = Qout + (int64_t)Np * 5 * e + n;
 #pragma unroll
             for (int s = 0; s < 5; ++s) {
-                const double dd = (double)T[s] + tb[Np * s];
-                qo[Np * s] = q[s] + bdt * dd;
+                const double dd = (double)T[s] + dq[s];
+                qo[Np * s] = sQ[s][n] + bdt * dd;
                 tb[Np * s] = dd * rka;
             }
         }

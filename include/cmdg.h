@@ -179,8 +179,15 @@ int cmdg_aux(cmdg_ctx *ctx, double **aux_device);
 int cmdg_gradient_flux(cmdg_ctx *ctx, double **gradflux_device, int32_t *ngradflux);
 
 /* Device pointer to the FP32 gradient flux of the last mixed-precision RHS
- * (Np, ngradflux, nelem): 9 (constant viscosity), 10 (Smagorinsky-Lilly) or 11
- * (learned closure: entry 10 holds the learned nu_0). */
+ * (Np, ngradflux, nelem), in stress form: the closure (constant viscosity,
+ * Smagorinsky-Lilly or the learned nu_0) folded into what the second-order flux
+ * needs.  Planes: tau, the viscous stress factor -2 nu_d S_d. (+ (2 nu / 3) tr S on
+ * the diagonal with divergence); e_d = (nu_d / Pr_t) dh_tot/dx_d (3); the viscosity.
+ * ngradflux 10 when the viscosity is isotropic (no buoyancy correction): tau as 6
+ * symmetric planes xx xy xz yy yz zz, e at 6-8, nu at 9; 15 otherwise: tau as 9
+ * row-major planes, e at 9-11, nu_d at 12-14.  Row d of the second-order flux is
+ * (0, rho tau_d., tau_d . rho u - rho e_d) (tendencies_momentum.jl:37-42,
+ * tendencies_energy.jl:27-56). */
 int cmdg_gradient_flux_f32(cmdg_ctx *ctx, float **gradflux_device, int32_t *ngradflux);
 
 /* ---------------------------------------------------------------------------

@@ -76,7 +76,7 @@ def _tgv_state(prm, grid, seed=0):
 
 def test_learned_rhs_matches_oracle():
     import torch
-    from cmdg.dg import to_device, to_host
+    from cmdg.dg import stress_form_layout, to_device, to_host
     print("[closure] learned rhs", flush=True)
     prm, grid = _tgv_case(3)
     Q0 = _tgv_state(prm, grid)
@@ -94,7 +94,8 @@ def test_learned_rhs_matches_oracle():
     got = to_host(dQ)
     errs = [rel_l2(got[:, s], ref[:, s]) for s in range(5)]
     print("  rel L2 per component: " + " ".join(f"{r:.2e}" for r in errs), flush=True)
-    nu_gpu = to_host(dg.state_gradient_flux_f32)[:, 10]
+    gfl = to_host(dg.state_gradient_flux_f32)
+    nu_gpu = gfl[:, stress_form_layout(gfl.shape[1])["nu"][0]]
     nu_ref = orc.gradient_flux()[:, 19]
     assert (nu_ref > 0).all()
     nerr = np.abs(nu_gpu - nu_ref) / nu_ref

@@ -263,6 +263,56 @@ def test_mixed_rhs_error(name):
     dg.close()
 
 
+# energy entry of the stress form: measured 1.9e-3 on the isentropic vortex (h_tot varies
+# by O(u^2) = O(25) J/kg across an element), 5e-5 .. 9e-5 on the other cases
+STRESS_FORM_ENERGY_RTOL = 5e-3
+
+
+@pytest.mark.parametrize("name", ["viscous_periodic", "tgv_kinematic", "tgv_smagorinsky", "les_walls"])
+def test_mixed_stress_form(name):
+    """The mixed gradient flux in stress form (tau, e, nu per node) against the oracle's
+    second-order flux at every node: row d is (0, rho tau_d., tau_d . rho u - rho e_d).
+    Isotropic layout without a buoyancy correction (constant viscosity with divergence,
+    Smagorinsky-Lilly without orientation), row-major with one (the LES case).  Relative
+    L2 per flux entry over all nodes, to the FP32 precision of the gradient."""
+    import torch
+    from cmdg.dg import DGModel, stress_form_layout, to_device, to_host
+    print(f"[mixed] stress form {name}", flush=True)
+    cfg, grid, aux, Q0 = _case(name)
+    orc = O.Oracle(cfg, grid, aux)
+    orc.tendency(Q0)
+    gf_ref = orc.gradient_flux()
+    dg = DGModel(cfg, grid, aux=aux, precision=abi.MIXED)
+    Q = to_device(Q0)
+    dg(torch.zeros_like(Q), Q, None, 0.0)
+    sf = to_host(dg.state_gradient_flux_f32).astype(np.float64)
+    dg.close()
+    lay = stress_form_layout(sf.shape[1])
+    assert lay["iso"] == (name != "les_walls")
+    L = cfg.layout
+    Np, E = grid.Np, grid.nelem
+    ref = np.zeros((Np, E, 3, 5))
+    got = np.zeros((Np, E, 3, 5))
+    for e in range(E):
+        for n in range(Np):
+            gPhi = aux[n, L["gradPhi"]:L["gradPhi"] + 3, e] if "gradPhi" in L else (0.0, 0.0, 0.0)
+            Delta = aux[n, L["Delta"], e] if "Delta" in L else 0.0
+            ref[n, e] = orc.flux_second_order(Q0[n, :, e], gf_ref[n, :, e], gPhi, Delta)
+            rho, m = Q0[n, 0, e], Q0[n, 1:4, e]
+            for d in range(3):
+                tau = sf[n, lay["tau"][d], e]
+                got[n, e, d, 1:4] = rho * tau
+                got[n, e, d, 4] = tau @ m - rho * sf[n, lay["e"][d], e]
+    orc.close()
+    errs = [rel_l2(got[..., s], ref[..., s]) for s in range(1, 5)]
+    print("  rel L2 per flux entry (momentum 1-3, energy): " + " ".join(f"{r:.2e}" for r in errs), flush=True)
+    for r in errs[:3]:
+        assert r < MIXED_RHS_RTOL
+    # the energy entry carries grad h_tot from the FP32 gradient argument: h_tot (about
+    # 3e5 J/kg) rounded at 2^-24 against its variation across an element
+    assert errs[3] < STRESS_FORM_ENERGY_RTOL
+
+
 def test_mixed_lsrk_steps_error():
     """Ten fused mixed-precision LSRK54 steps of the LES case against ten FP64 steps:
     relative L2 error of the change of each state component."""
