@@ -730,12 +730,17 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
             }
             // tiled traversal (CMDG_ELEMENT_TILE=T sets the tile edge, 0 keeps the storage
             // order; CMDG_TILED_PASSES = tendency, gradient, both or none chooses the
-            // passes that walk the tiles).  Default: both passes for mixed contexts, the
-            // tendency pass alone for FP64 ones.  Measured at 96^3 on one box: mixed +
-            // learned closure 55.0 (tendency pass) -> 56.0 GDOF/s (both; tile edge 6 / 12 /
-            // 16: 55.3 / 56.2 / 55.8), mixed + Smagorinsky-Lilly 59.6 -> 60.3; FP64 40.4
-            // (tendency pass) against 38.6 (both): the FP64 gradient pass, which writes its
-            // second-order face trace, loses in tiles more than its gathers gain.
+            // passes that walk the tiles).  Default: the tendency pass walks the tiles; the
+            // mixed gradient pass too when, in storage order, its face neighbours' state
+            // would have left one XCD's L2 before it gathers it (the 90th percentile of the
+            // storage distance between face neighbours, in state bytes, above 4 MiB).
+            // Measured on one box: 96^3 (distance 9216 elements, 46 MB) mixed + learned
+            // closure 55.0 (tendency pass) -> 56.0 GDOF/s (both; tile edge 6 / 12 / 16:
+            // 55.3 / 56.2 / 55.8), mixed + Smagorinsky-Lilly 59.6 -> 60.3; 25^3 (625
+            // elements, 3.1 MB) 50.3 (tendency pass) against 49.3 (both), 54.0 against 52.8.
+            // FP64 at 96^3: 40.4 (tendency pass) against 38.6 (both): its gradient pass,
+            // which writes the second-order face trace, loses in tiles more than its
+            // gathers gain.
             const char *tile_env = std::getenv("CMDG_ELEMENT_TILE");
             const int T = tile_env ? std::atoi(tile_env) : 8;
             std::vector<int32_t> order;
@@ -743,7 +748,19 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
             if (brick ? tiled_order_lattice(lat, nax, E, T, order) : tiled_order(d, Np, T, order)) {
                 if ((rc = upload(c, &c->d_order, order.data(), order.size()))) return rc;
                 const char *penv = std::getenv("CMDG_TILED_PASSES");
-                const std::string passes = penv ? penv : (d->precision == CMDG_MIXED ? "both" : "tendency");
+                std::string passes = penv ? penv : "tendency";
+                if (!penv && d->precision == CMDG_MIXED) {
+                    std::vector<int64_t> gap;
+                    gap.reserve(nbe.size());
+                    for (int64_t e = 0; e < E; ++e)
+                        for (int f = 0; f < 6; ++f)
+                            if (nbe[6 * e + f] >= 0) gap.push_back(std::llabs((int64_t)nbe[6 * e + f] - e));
+                    if (!gap.empty()) {
+                        auto q = gap.begin() + (int64_t)(0.9 * (double)(gap.size() - 1));
+                        std::nth_element(gap.begin(), q, gap.end());
+                        if ((double)*q * Np * 5 * sizeof(double) > 4.0 * 1024 * 1024) passes = "both";
+                    }
+                }
                 if (passes == "both" || passes == "gradient") c->ord_grad = c->d_order;
                 if (passes == "both" || passes == "tendency") c->ord_tend = c->d_order;
             }
