@@ -356,12 +356,12 @@ bool static_classes(const cmdg_desc *d, const KParams &P, int Np, std::vector<in
 // positions away.  The lattice position of each element comes from its centroid
 // (vgeo x1, x2, x3); returns false, keeping the storage order, when the centroids do
 // not form a complete lattice or the descriptor has no coordinates.
-bool tiled_order_lattice(const std::vector<int64_t> idx[3], const int64_t nax[3], int64_t E, int T,
+bool tiled_order_lattice(const std::vector<int64_t> idx[3], const int64_t nax[3], int64_t E, const int T[3],
                          std::vector<int32_t> &order);
 
-bool tiled_order(const cmdg_desc *d, int Np, int T, std::vector<int32_t> &order) {
+bool tiled_order(const cmdg_desc *d, int Np, const int T[3], std::vector<int32_t> &order) {
     const int64_t E = d->nelem;
-    if (T <= 0 || d->nvgeo < 15 || E > INT32_MAX) return false;
+    if (T[0] < 0 || T[1] < 0 || T[2] < 0 || d->nvgeo < 15 || E > INT32_MAX) return false;
     std::vector<int64_t> idx[3];
     int64_t nax[3];
     for (int a = 0; a < 3; ++a) {
@@ -390,11 +390,18 @@ bool tiled_order(const cmdg_desc *d, int Np, int T, std::vector<int32_t> &order)
 }
 
 // The tile order of elements at lattice positions idx[a][e] of an nax[0] x nax[1] x
-// nax[2] lattice (false when the positions do not fill the lattice once each).
-bool tiled_order_lattice(const std::vector<int64_t> idx[3], const int64_t nax[3], int64_t E, int T,
+// nax[2] lattice, tiles of T[0] x T[1] x T[2] elements (0: the whole axis), z fastest
+// then x then y within a tile and between tiles (false when the positions do not fill
+// the lattice once each).
+bool tiled_order_lattice(const std::vector<int64_t> idx[3], const int64_t nax[3], int64_t E, const int T[3],
                          std::vector<int32_t> &order) {
-    if (T <= 0 || E > INT32_MAX || nax[0] * nax[1] * nax[2] != E) return false;
-    const int64_t ntx = (nax[0] + T - 1) / T, ntz = (nax[2] + T - 1) / T;
+    if (E > INT32_MAX || nax[0] * nax[1] * nax[2] != E) return false;
+    int64_t t[3];
+    for (int a = 0; a < 3; ++a) {
+        if (T[a] < 0) return false;
+        t[a] = T[a] == 0 ? nax[a] : std::min<int64_t>(T[a], nax[a]);
+    }
+    const int64_t ntx = (nax[0] + t[0] - 1) / t[0], ntz = (nax[2] + t[2] - 1) / t[2];
     std::vector<std::pair<int64_t, int32_t>> key(E);
     std::vector<char> seen(E, 0);
     for (int64_t e = 0; e < E; ++e) {
@@ -402,8 +409,8 @@ bool tiled_order_lattice(const std::vector<int64_t> idx[3], const int64_t nax[3]
         const int64_t cell = iz + nax[2] * (ix + nax[0] * iy);
         if (seen[cell]) return false;  // two elements at one lattice point
         seen[cell] = 1;
-        const int64_t tile = iz / T + ntz * (ix / T + ntx * (iy / T));
-        key[e] = {((tile * T + iy % T) * T + ix % T) * T + iz % T, (int32_t)e};
+        const int64_t tile = iz / t[2] + ntz * (ix / t[0] + ntx * (iy / t[1]));
+        key[e] = {((tile * t[1] + iy % t[1]) * t[0] + ix % t[0]) * t[2] + iz % t[2], (int32_t)e};
     }
     std::sort(key.begin(), key.end());
     order.resize(E);
@@ -728,39 +735,51 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
             } else {
                 crep.clear();
             }
-            // tiled traversal (CMDG_ELEMENT_TILE=T sets the tile edge, 0 keeps the storage
-            // order; CMDG_TILED_PASSES = tendency, gradient, both or none chooses the
-            // passes that walk the tiles).  Default: the tendency pass walks the tiles; the
-            // mixed gradient pass too when, in storage order, its face neighbours' state
-            // would have left one XCD's L2 before it gathers it (the 90th percentile of the
-            // storage distance between face neighbours, in state bytes, above 4 MiB).
-            // Measured on one box: 96^3 (distance 9216 elements, 46 MB) mixed + learned
-            // closure 55.0 (tendency pass) -> 56.0 GDOF/s (both; tile edge 6 / 12 / 16:
-            // 55.3 / 56.2 / 55.8), mixed + Smagorinsky-Lilly 59.6 -> 60.3; 25^3 (625
-            // elements, 3.1 MB) 50.3 (tendency pass) against 49.3 (both), 54.0 against 52.8.
-            // FP64 at 96^3: 40.4 (tendency pass) against 38.6 (both): its gradient pass,
-            // which writes the second-order face trace, loses in tiles more than its
-            // gathers gain.
+            // tiled traversal.  CMDG_ELEMENT_TILE=T sets cubic tiles of edge T (0 keeps
+            // the storage order) and "TX,TY,TZ" tile extents along x, y, z (0: the whole
+            // axis); CMDG_TILED_PASSES = tendency, gradient, both or none chooses the passes
+            // that walk the tiles.  Default: the tendency pass walks 8^3 tiles.  A mixed
+            // context whose face neighbours would, in storage order, leave one XCD's L2
+            // before the gradient pass gathers their state (the 90th percentile of the
+            // storage distance between face neighbours, in state bytes, above 4 MiB) walks
+            // 12 x 12 columns of elements spanning the whole y axis, in both passes.
+            // Measured on one box, chosen form (mixed + learned closure):
+            //   96^3 (distance 9216 elements, 46 MB): tendency pass in 8^3 tiles 55.0 GDOF/s;
+            //     both passes in 8^3 tiles 55.7-56.0 (edge 6 / 12 / 16: 55.3 / 56.2 / 55.8);
+            //     both in 12 x 12 columns 56.6 (10 x 10: 56.1, 14 x 14: 55.8, 8 x 8: 56.1);
+            //     mixed + Smagorinsky-Lilly 59.6 / 60.3 / 60.3;
+            //   25^3 (625 elements, 3.1 MB): tendency pass in 8^3 tiles 50.0-50.3; both
+            //     passes 49.3; 12 x 12 columns 49.2 (tendency pass), 48.6 (both).
+            // FP64 at 96^3: 40.1-40.4 with the tendency pass in 8^3 tiles (12 x 12 columns
+            // 40.2), 37.6-38.6 with both: its gradient pass, which writes the second-order
+            // face trace, loses in tiles more than its gathers gain.
+            bool far = false;
+            if (d->precision == CMDG_MIXED) {
+                std::vector<int64_t> gap;
+                gap.reserve(nbe.size());
+                for (int64_t e = 0; e < E; ++e)
+                    for (int f = 0; f < 6; ++f)
+                        if (nbe[6 * e + f] >= 0) gap.push_back(std::llabs((int64_t)nbe[6 * e + f] - e));
+                if (!gap.empty()) {
+                    auto q = gap.begin() + (int64_t)(0.9 * (double)(gap.size() - 1));
+                    std::nth_element(gap.begin(), q, gap.end());
+                    far = (double)*q * Np * 5 * sizeof(double) > 4.0 * 1024 * 1024;
+                }
+            }
             const char *tile_env = std::getenv("CMDG_ELEMENT_TILE");
-            const int T = tile_env ? std::atoi(tile_env) : 8;
+            int T[3] = {8, 8, 8};
+            if (far) T[0] = 12, T[1] = 0, T[2] = 12;
+            if (tile_env && std::strchr(tile_env, ',')) {
+                if (std::sscanf(tile_env, "%d,%d,%d", &T[0], &T[1], &T[2]) != 3) T[0] = T[1] = T[2] = -1;
+            } else if (tile_env) {
+                T[0] = T[1] = T[2] = std::atoi(tile_env) > 0 ? std::atoi(tile_env) : -1;
+            }
             std::vector<int32_t> order;
             const int64_t nax[3] = {brick ? brick->ne[0] : 0, brick ? brick->ne[1] : 0, brick ? brick->ne[2] : 0};
             if (brick ? tiled_order_lattice(lat, nax, E, T, order) : tiled_order(d, Np, T, order)) {
                 if ((rc = upload(c, &c->d_order, order.data(), order.size()))) return rc;
                 const char *penv = std::getenv("CMDG_TILED_PASSES");
-                std::string passes = penv ? penv : "tendency";
-                if (!penv && d->precision == CMDG_MIXED) {
-                    std::vector<int64_t> gap;
-                    gap.reserve(nbe.size());
-                    for (int64_t e = 0; e < E; ++e)
-                        for (int f = 0; f < 6; ++f)
-                            if (nbe[6 * e + f] >= 0) gap.push_back(std::llabs((int64_t)nbe[6 * e + f] - e));
-                    if (!gap.empty()) {
-                        auto q = gap.begin() + (int64_t)(0.9 * (double)(gap.size() - 1));
-                        std::nth_element(gap.begin(), q, gap.end());
-                        if ((double)*q * Np * 5 * sizeof(double) > 4.0 * 1024 * 1024) passes = "both";
-                    }
-                }
+                const std::string passes = penv ? penv : (far ? "both" : "tendency");
                 if (passes == "both" || passes == "gradient") c->ord_grad = c->d_order;
                 if (passes == "both" || passes == "tendency") c->ord_tend = c->d_order;
             }

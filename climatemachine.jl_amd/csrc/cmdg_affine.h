@@ -289,10 +289,14 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
     for (int s = 0; s < 5; ++s) q[s] = sQ[s][vn];
     const double Phi = sQ[5][vn], ref_p = sQ[6][vn];
     double Php = Phi, refpp = ref_p;
+    double f2p[TR_F2] = {};  // the neighbour's second-order row, loaded with its state
     if (!wall) {
         const double *qb = Q + (int64_t)Np * 5 * ne + face_node<Nq>(fo, fn);
 #pragma unroll
         for (int s = 0; s < 5; ++s) qp[s] = qb[Np * s];
+        if (DIFF)
+#pragma unroll
+            for (int v = 0; v < TR_F2; ++v) f2p[v] = f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)];
         if (!A.aux_cont) {
             Php = A.str[tri<Nfp>(ne, fo, 0, TR_S, fn)];
             refpp = A.str[tri<Nfp>(ne, fo, 1, TR_S, fn)];
@@ -326,7 +330,7 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
 #pragma unroll
         for (int v = 0; v < TR_F2; ++v) {
             Fm[1 + v] = f2own[v];  // the same bits the gradient kernel put in the trace
-            Fp[1 + v] = f2tr[tri<Nfp>(ne, fo, v, TR_F2, fn)];
+            Fp[1 + v] = f2p[v];
         }
 #pragma unroll
         for (int s = 0; s < 5; ++s) flux[s] += (Fm[s] + Fp[s]) * h;
@@ -477,10 +481,11 @@ k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
 #pragma unroll
             for (int s = 0; s < 5; ++s) tb[Np * s] = T[s];
         } else {
+            // the stage's state from its LDS copy (sQ is intact after the face tasks)
             double *qo = Qout + (int64_t)Np * 5 * e + n;
 #pragma unroll
             for (int s = 0; s < 5; ++s) {
-                qo[Np * s] = q[s] + bdt * T[s];
+                qo[Np * s] = sQ[s][n] + bdt * T[s];
                 tb[Np * s] = T[s] * rka;
             }
         }

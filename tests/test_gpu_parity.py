@@ -556,15 +556,16 @@ def test_static_aux_without_classes(precision):
 def test_traversal_orders_agree(precision):
     """Each block computes its own element from inputs no launch writes, so the order
     in which the launches walk the elements changes no bit: storage order
-    (CMDG_ELEMENT_TILE=0) against 2 x 2 x 2 tiles on the 3 x 3 x 3 LES brick (the default
-    8 x 8 x 8 tile covers it whole and walks it in storage order), three fused steps."""
+    (CMDG_ELEMENT_TILE=0) against 2 x 2 x 2 tiles and against 2 x 2 columns spanning the
+    y axis ("2,0,2") on the 3 x 3 x 3 LES brick (the default 8 x 8 x 8 tile covers it whole
+    and walks it in storage order), three fused steps."""
     import os
     from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
     print(f"[parity] traversal orders {precision}", flush=True)
     cfg, grid, aux, Q0 = _case("les_walls")
     prec = abi.FP64 if precision == "fp64" else abi.MIXED
     out = {}
-    for tile in ("0", "2"):
+    for tile in ("0", "2", "2,0,2"):
         os.environ["CMDG_ELEMENT_TILE"] = tile
         os.environ["CMDG_TILED_PASSES"] = "both"
         try:
@@ -579,8 +580,9 @@ def test_traversal_orders_agree(precision):
         out[tile] = (to_host(Q), to_host(dg.state_auxiliary))
         dg.close()
     assert np.isfinite(out["0"][0]).all()
-    assert np.array_equal(out["0"][0], out["2"][0])
-    assert np.array_equal(out["0"][1], out["2"][1])
+    for tile in ("2", "2,0,2"):
+        assert np.array_equal(out["0"][0], out[tile][0]), tile
+        assert np.array_equal(out["0"][1], out[tile][1]), tile
 
 
 @pytest.mark.parametrize("variant", ["fp64_affine", "fp64_general", "mixed", "euler_affine"])
