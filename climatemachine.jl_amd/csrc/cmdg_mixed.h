@@ -96,10 +96,12 @@ template <bool ISO> struct StressForm {
 };
 
 // Stress form of one node from its gradient flux gf = [grad h_tot (3), S (6), N2, ...]
-// and viscosity nu[3]; every operation rounded (no contraction), in the order the row
-// below and the face neighbour's row use them.
+// and viscosity nu[3], plane p stored at out[p * stride] as soon as it is formed; every
+// operation rounded (no contraction), in the order the row below and the face
+// neighbour's row use them.
 template <bool SMAG, bool ISO>
-__device__ __forceinline__ void stress_form_f(const MixParams &P, const float *gf, const float nu[3], float *out) {
+__device__ __forceinline__ void stress_form_f(const MixParams &P, const float *gf, const float nu[3], float *out,
+                                              int stride) {
 #pragma clang fp contract(off)
     using L = StressForm<ISO>;
     const float *S = gf + 3;
@@ -115,15 +117,15 @@ __device__ __forceinline__ void stress_form_f(const MixParams &P, const float *g
             const int a = d < c ? d : c, b = d < c ? c : d;
             float t = m2 * S[a == 0 ? b : (a == 1 ? 2 + b : 5)];
             if (!SMAG && P.with_div && c == d) t += dv;
-            out[L::tau(d, c)] = t;
+            out[stride * L::tau(d, c)] = t;
         }
-        out[L::E + d] = (nud * P.inv_Pr) * gf[d];
+        out[stride * (L::E + d)] = (nud * P.inv_Pr) * gf[d];
     }
     if (ISO) {
-        out[L::NU] = nu[0];
+        out[stride * L::NU] = nu[0];
     } else {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) out[L::NU + d] = nu[d];
+        for (int d = 0; d < 3; ++d) out[stride * (L::NU + d)] = nu[d];
     }
 }
 
@@ -188,15 +190,18 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 }
 
 // ------------------------------------------------------------------ gradient (mixed)
-// Five waves per SIMD with a gravitational potential, six without at Nq = 5 (its theta_v
-// work gone; the register caps this asks for, 96 and 80, spill nothing there): the
-// Smagorinsky-Lilly variant waits on memory more than it computes (the learned one is
-// VALU-bound).
+// At least six waves per SIMD at Nq = 5 (a cap of 80 registers, no spills): the
+// derivative and the lifted jumps run a field at a time and the stress form is stored as it
+// is formed, which keeps the learned variants at 70 registers (7 waves) and the
+// Smagorinsky-Lilly ones at 52-57; a whole-gradient-then-lifting order needed 79-94 (5-6
+// waves) and spilled under this cap with a gravitational potential.  Measured on one box
+// (whole order -> field order): 96^3 chosen form 56.8 -> 57.1 GDOF/s (gradient 4036 ->
+// 3975 us), mixed Smagorinsky-Lilly 60.6 -> 61.9; 25^3 50.1 -> 50.4 (86.7 -> 84.5 us).
 // BUOY: the context has a gravitational potential (FlatOrientation), so Smagorinsky-Lilly's
 // N^2 = grad theta_v . grad Phi / theta_v needs theta_v and its gradient; without one
 // (grad Phi = 0, N^2 = 0 exactly) the kernel evaluates neither.
 template <int Nq, bool SMAG, bool LEARN, bool BUOY>
-__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu((Nq == 5 && !BUOY) ? 6 : 5)))
+__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(Nq == 5 ? 6 : 5)))
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
                double *__restrict__ aux, float *__restrict__ gfo, int write_aux, const MlpDev *__restrict__ mlp) {
 #pragma clang fp contract(fast)
@@ -290,6 +295,18 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     float g[3][5] = {};
     float gf[11];
     if (act) {
+        // per field: the volume derivative along r, s, t, then the lifted central jumps of
+        // this node's faces, M^-1 Mf n (G* - G-) (a field at a time keeps fewer values live)
+        const float MI = (float)(1.0 / (ge[3] * A.W[n]));
+        int fl[3];  // this node's face slot along each axis (-1: interior along it)
+        float wl[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int fn;
+            const int f = node_face_dir<Nq>(i, j, k, d, fn);
+            fl[d] = f < 0 ? -1 : f * Nfp + fn;
+            wl[d] = f < 0 ? 0.0f : MI * (float)(ge[4 + f] * A.Wf[fn]);
+        }
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
             float g1 = 0.0f, g2 = 0.0f, g3 = 0.0f;
@@ -304,17 +321,9 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             g[0][s] = xi0 * g1;
             g[1][s] = xi1 * g2;
             g[2][s] = xi2 * g3;
-        }
-        // lifted central jumps of this node's faces: M^-1 Mf n (G* - G-)
-        const float MI = (float)(1.0 / (ge[3] * A.W[n]));
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            int fn;
-            const int f = node_face_dir<Nq>(i, j, k, d, fn);
-            if (f < 0) continue;
-            const float w = MI * (float)(ge[4 + f] * A.Wf[fn]);
-#pragma unroll
-            for (int s = 0; s < 5; ++s) g[d][s] += w * sJ[f * Nfp + fn][s];
+            for (int d = 0; d < 3; ++d)
+                if (fl[d] >= 0) g[d][s] += wl[d] * sJ[fl[d]][s];
         }
         grad_flux_f<SMAG>(g, gPhi, thv, gf);
     }
@@ -332,12 +341,9 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     if (act) {
         // the stress form of the node's second-order flux: the tendency forms its rows,
         // and the face neighbours' rows, from it
-        float nu[3], sf[L::NGF];
+        float nu[3];
         viscosity_f<SMAG, LEARN>(MP, (float)q[0], gf, gPhi, (float)ge[10], nu);
-        stress_form_f<SMAG, ISO>(MP, gf, nu, sf);
-        float *go = gfo + (int64_t)Np * L::NGF * e + n;
-#pragma unroll
-        for (int s = 0; s < L::NGF; ++s) go[Np * s] = sf[s];
+        stress_form_f<SMAG, ISO>(MP, gf, nu, gfo + (int64_t)Np * L::NGF * e + n, Np);
     }
 }
 
