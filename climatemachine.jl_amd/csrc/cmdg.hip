@@ -704,7 +704,10 @@ int create_impl(const cmdg_desc *d, const cmdg_brick *brick, void *stream, cmdg_
             const char *tenv = std::getenv("CMDG_NEIGHBOUR_STATIC_TRACE");  // 1: always read it
             c->AA.aux_cont = (cont && !(tenv && tenv[0] == '1')) ? 1 : 0;
             if ((rc = upload(c, &c->d_str, st.data(), st.size()))) return rc;
-            if ((rc = upload<double>(c, &c->d_f2tr, nullptr, (size_t)E * 6 * TR_F2 * Nfp))) return rc;
+            // the FP64 second-order face trace (mixed contexts carry the stress form instead)
+            if (d->precision != CMDG_MIXED && c->diffusive &&
+                (rc = upload<double>(c, &c->d_f2tr, nullptr, (size_t)E * 6 * TR_F2 * Nfp)))
+                return rc;
             c->AA.geo = c->d_geo;
             c->AA.nbe = c->d_nbe;
             c->AA.W = c->d_W;
