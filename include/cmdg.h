@@ -311,13 +311,16 @@ int cmdg_not_finite_fields(cmdg_ctx *ctx, const double *Q, int32_t *mask);
  * many elements agree bitwise (every element layer of a brick whose reference state
  * depends on height alone), and the kernels read them from one shared block per class
  * instead of one per element; bit 3 = on those bricks, a pass walks the elements in
- * lattice tiles of T x T x T (z fastest, then x, then y; T = 8; by default the
- * tendency pass, the gradient pass keeping storage order) so that a block's face
- * neighbours ran shortly before it on the same XCD.
+ * lattice tiles (z fastest, then x, then y, within and between tiles) so that a
+ * block's face neighbours ran shortly before it on the same XCD: by default the
+ * tendency pass in 8 x 8 x 8 tiles, and a mixed context whose face neighbours lie
+ * farther apart in storage order than one XCD's L2 holds walks 12 x 12 element columns
+ * spanning y in both passes.
  * Setting CMDG_GENERAL_GEOMETRY=1 in the environment before cmdg_create clears bit 0;
  * CMDG_NEIGHBOUR_STATIC_TRACE=1 clears bit 1; CMDG_STATIC_AUX_CLASSES=0 clears bit 2;
- * CMDG_ELEMENT_TILE=T sets the tile edge (T = 0 clears bit 3) and CMDG_TILED_PASSES
- * (tendency, gradient, both, none) the passes that walk the tiles. */
+ * CMDG_ELEMENT_TILE=T sets cubic tiles of edge T (T = 0 clears bit 3) and
+ * CMDG_ELEMENT_TILE=TX,TY,TZ the tile extents per axis (0: the whole axis);
+ * CMDG_TILED_PASSES (tendency, gradient, both, none) the passes that walk the tiles. */
 int cmdg_geometry_kind(cmdg_ctx *ctx, int32_t *kind);
 
 /* Per-kernel device timing (diagnostic; no reference counterpart): when enabled,
