@@ -46,11 +46,17 @@ __device__ __forceinline__ float strain_norm_f(const float *gf) {
 }
 
 // Smagorinsky-Lilly viscosity; LEARN takes nu_0 from gf[10] (the learned closure).
-template <bool SMAG, bool LEARN = false>
+// BUOY false: no gravitational potential, so k = 0 and the buoyancy factor drops out:
+// nu_d = (nu_0 - 0) + 0 f_b = nu_0 exactly, formed without N^2, Ri or f_b.
+template <bool SMAG, bool LEARN = false, bool BUOY = true>
 __device__ __forceinline__ void viscosity_f(const MixParams &P, float rho, const float *gf, const float gPhi[3],
                                             float Delta, float nu[3]) {
 #pragma clang fp contract(off)
-    if (SMAG) {
+    if (SMAG && !BUOY) {
+        const float CD = P.C_smag * Delta;
+        const float nu0 = LEARN ? gf[10] : strain_norm_f(gf) * (CD * CD) + 1e-5f;
+        nu[0] = nu[1] = nu[2] = nu0;
+    } else if (SMAG) {
         const float normS = strain_norm_f(gf);
         const float ig = 1.0f / P.grav;
         const float k[3] = {gPhi[0] * ig, gPhi[1] * ig, gPhi[2] * ig};
@@ -140,8 +146,9 @@ __device__ __forceinline__ void flux2_row_stress(const float t[3], float e, cons
     F[4] = t[0] * q[1] + t[1] * q[2] + t[2] * q[3] - e * q[0];
 }
 
-// gradient flux of the 3 x 5 gradient (energy.jl:50-59, TurbulenceClosures.jl:455-470)
-template <bool SMAG>
+// gradient flux of the 3 x 5 gradient (energy.jl:50-59, TurbulenceClosures.jl:455-470);
+// BUOY false: no gravitational potential, N^2 = 0
+template <bool SMAG, bool BUOY = true>
 __device__ __forceinline__ void grad_flux_f(const float g[3][5], const float gPhi[3], float thv, float *gf) {
 #pragma clang fp contract(fast)
     gf[0] = g[0][3];
@@ -153,7 +160,7 @@ __device__ __forceinline__ void grad_flux_f(const float g[3][5], const float gPh
     gf[6] = g[1][1];
     gf[7] = 0.5f * (g[2][1] + g[1][2]);
     gf[8] = g[2][2];
-    if (SMAG) gf[9] = (g[0][4] * gPhi[0] + g[1][4] * gPhi[1] + g[2][4] * gPhi[2]) / thv;
+    if (SMAG) gf[9] = BUOY ? (g[0][4] * gPhi[0] + g[1][4] * gPhi[1] + g[2][4] * gPhi[2]) / thv : 0.0f;
 }
 
 struct ThermoD {
@@ -190,18 +197,20 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 }
 
 // ------------------------------------------------------------------ gradient (mixed)
-// At least six waves per SIMD at Nq = 5 (a cap of 80 registers, no spills): the
-// derivative and the lifted jumps run a field at a time and the stress form is stored as it
-// is formed, which keeps the learned variants at 70 registers (7 waves) and the
-// Smagorinsky-Lilly ones at 52-57; a whole-gradient-then-lifting order needed 79-94 (5-6
-// waves) and spilled under this cap with a gravitational potential.  Measured on one box
-// (whole order -> field order): 96^3 chosen form 56.8 -> 57.1 GDOF/s (gradient 4036 ->
-// 3975 us), mixed Smagorinsky-Lilly 60.6 -> 61.9; 25^3 50.1 -> 50.4 (86.7 -> 84.5 us).
+// Seven waves per SIMD at Nq = 5 (a cap of 72 registers, no spills): the derivative and
+// the lifted jumps run a field at a time and the stress form is stored as it is formed,
+// which keeps the learned variants at 66-70 registers and the Smagorinsky-Lilly ones at
+// 49-57; a whole-gradient-then-lifting order needed 79-94 (5-6 waves) and spilled under an
+// 80 cap with a gravitational potential.  Measured on one box (whole order -> field
+// order): 96^3 chosen form 56.8 -> 57.1 GDOF/s (gradient 4036 -> 3975 us), mixed
+// Smagorinsky-Lilly 60.6 -> 61.9; 25^3 50.1 -> 50.4 (86.7 -> 84.5 us).  Without a
+// gravitational potential the kernel also skips N^2, Ri and the buoyancy factor (exactly
+// nu_d = nu_0 there) and theta_v's jumps: 96^3 gradient 3972 -> 3913 us, bit for bit.
 // BUOY: the context has a gravitational potential (FlatOrientation), so Smagorinsky-Lilly's
 // N^2 = grad theta_v . grad Phi / theta_v needs theta_v and its gradient; without one
 // (grad Phi = 0, N^2 = 0 exactly) the kernel evaluates neither.
 template <int Nq, bool SMAG, bool LEARN, bool BUOY>
-__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(Nq == 5 ? 6 : 5)))
+__global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(Nq == 5 ? 7 : 5)))
 k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
                double *__restrict__ aux, float *__restrict__ gfo, int write_aux, const MlpDev *__restrict__ mlp) {
 #pragma clang fp contract(fast)
@@ -287,7 +296,8 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             for (int s = 0; s < 5; ++s) jump[s] = nd * (Gp[s] - Gm[s]);
         }
 #pragma unroll
-        for (int s = 0; s < 5; ++s) sJ[t][s] = jump[s];
+        for (int s = 0; s < 5; ++s)
+            if (s < 4 || buoy) sJ[t][s] = jump[s];
     }
     __syncthreads();
 
@@ -321,11 +331,12 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             g[0][s] = xi0 * g1;
             g[1][s] = xi1 * g2;
             g[2][s] = xi2 * g3;
+            if (s < 4 || buoy)  // theta_v's gradient only where N^2 needs it
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
-                if (fl[d] >= 0) g[d][s] += wl[d] * sJ[fl[d]][s];
+                for (int d = 0; d < 3; ++d)
+                    if (fl[d] >= 0) g[d][s] += wl[d] * sJ[fl[d]][s];
         }
-        grad_flux_f<SMAG>(g, gPhi, thv, gf);
+        grad_flux_f<SMAG, buoy>(g, gPhi, thv, gf);
     }
     if (LEARN) {
         // the network runs on the assembled gradient (volume + lifted face jumps):
@@ -342,7 +353,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
         // the stress form of the node's second-order flux: the tendency forms its rows,
         // and the face neighbours' rows, from it
         float nu[3];
-        viscosity_f<SMAG, LEARN>(MP, (float)q[0], gf, gPhi, (float)ge[10], nu);
+        viscosity_f<SMAG, LEARN, buoy>(MP, (float)q[0], gf, gPhi, (float)ge[10], nu);
         stress_form_f<SMAG, ISO>(MP, gf, nu, gfo + (int64_t)Np * L::NGF * e + n, Np);
     }
 }
