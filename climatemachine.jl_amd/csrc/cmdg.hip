@@ -940,7 +940,11 @@ int cmdg_lsrk54_steps(cmdg_ctx *c, double *Q, double *dQ, double t, double dt, i
     for (int64_t step = 0; step < nsteps; ++step)
         for (int s = 0; s < 5; ++s) {
             const double a = RKA[(s + 1) % 5], bdt = RKB[s] * dt;
-            const int wa = (s == 4);  // aux keeps the state of the last stage, as after dostep!
+            // aux keeps theta_v and air_T of the last stage's state, as after dostep!.  No
+            // kernel reads them back (each forms them from the state), so only the last
+            // step's write reaches the caller and the earlier steps skip it (theta_v costs
+            // a power per node).
+            const int wa = (s == 4 && step == nsteps - 1);
             if (c->diffusive) {
                 const bool first = odd && step == 0 && s == 0 && !c->affine;
                 if ((rc = launch_gradient(c, first ? C : src, first ? S : nullptr, wa))) return rc;

@@ -163,12 +163,27 @@ __device__ __forceinline__ void grad_flux_f(const float g[3][5], const float gPh
     if (SMAG) gf[9] = BUOY ? (g[0][4] * gPhi[0] + g[1][4] * gPhi[1] + g[2][4] * gPhi[2]) / thv : 0.0f;
 }
 
+// FP64 reciprocal of a positive normal x: the hardware estimate refined by two Newton
+// steps (an error at most about one FP64 ulp, against a correctly rounded quotient; the
+// FP32 values the mixed kernels round from it are the same but where an FP64 result lies
+// within that ulp of an FP32 rounding boundary).  Five FP64 instructions where the IEEE
+// division takes eleven (its range scaling and fix-ups serve zero, denormal and huge
+// operands, which a density or a mass-matrix entry is not).  Every mixed kernel takes
+// its 1/rho and M^-1 from here, so both sides of a face get the same bits.
+__device__ __forceinline__ double recip_d(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 struct ThermoD {
     double temp, p;
 };
 __device__ __forceinline__ ThermoD thermo_d(const KParams &P, const double q[5], double Phi) {
 #pragma clang fp contract(fast)
-    const double rinv = 1.0 / q[0];
+    const double rinv = recip_d(q[0]);
     const double e_int = rinv * (q[4] - 0.5 * rinv * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) - q[0] * Phi);
     ThermoD t;
     t.temp = P.T_0 + e_int * P.inv_cv;
@@ -186,7 +201,7 @@ template <bool SMAG>
 __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M, const double q[5], const ThermoD &t,
                                            bool buoy, float G[5]) {
 #pragma clang fp contract(fast)
-    const double rinv = 1.0 / q[0];
+    const double rinv = recip_d(q[0]);
     G[0] = (float)(rinv * q[1]);
     G[1] = (float)(rinv * q[2]);
     G[2] = (float)(rinv * q[3]);
@@ -307,7 +322,7 @@ k_gradient_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     if (act) {
         // per field: the volume derivative along r, s, t, then the lifted central jumps of
         // this node's faces, M^-1 Mf n (G* - G-) (a field at a time keeps fewer values live)
-        const float MI = (float)(1.0 / (ge[3] * A.W[n]));
+        const float MI = (float)recip_d(ge[3] * A.W[n]);
         int fl[3];  // this node's face slot along each axis (-1: interior along it)
         float wl[3];
 #pragma unroll
@@ -526,7 +541,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             for (int s = 0; s < L::NREAD; ++s) sf[s] = gb[Np * s];
         }
         const double Md = ge[3] * A.W[n];
-        MI = (float)(1.0 / Md);
+        MI = (float)recip_d(Md);
         const float M = (float)Md;
 #pragma unroll
         for (int ax = 0; ax < 3; ++ax) {

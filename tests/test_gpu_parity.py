@@ -589,8 +589,11 @@ def test_traversal_orders_agree(precision):
 def test_multistep_matches_single_steps(variant):
     """cmdg_lsrk54_steps (the solve! loop without callbacks; the stages alternate the
     state between two buffers across the steps of the call) gives bit for bit the state
-    of the same number of single dostep! calls."""
+    of the same number of single dostep! calls, and the same auxiliary theta_v and air_T
+    (those of the last stage's state, written once, by the call's last step: they are
+    cleared to NaN before the call, so the check sees that write)."""
     import os
+    import torch
     from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
     print(f"[parity] multistep {variant}", flush=True)
     cfg, grid, aux, Q0 = _case("euler_periodic" if variant == "euler_affine" else "les_walls")
@@ -606,6 +609,13 @@ def test_multistep_matches_single_steps(variant):
     sa = LSRK54CarpenterKennedy(dg, Qa, dt=dt, t0=0.0)
     for k in range(4):
         sa.dostep(Qa, None, k * dt)
+    lay = cfg.aux_layout()
+    outs = [lay["theta_v"], lay["air_T"]]
+    aux_a = to_host(dg.state_auxiliary)
+    with torch.no_grad():
+        for f in outs:
+            dg.state_auxiliary[:, f, :] = float("nan")
+    torch.cuda.synchronize()
     Qb = to_device(Q0)
     sb = LSRK54CarpenterKennedy(dg, Qb, dt=dt, t0=0.0)
     solve(Qb, sb, numberofsteps=4, timeend=math.inf)
@@ -613,6 +623,10 @@ def test_multistep_matches_single_steps(variant):
     assert np.isfinite(a).all()
     assert not np.array_equal(a, Q0)
     assert np.array_equal(a, b)
+    aux_b = to_host(dg.state_auxiliary)
+    for f in outs:
+        assert np.isfinite(aux_b[:, f, :]).all(), f
+    assert np.array_equal(aux_a, aux_b)
     assert abs(sb.gettime() - 4 * dt) < 1e-15 and sb.steps == 4
     dg.close()
 
