@@ -202,13 +202,13 @@ int launch_gradient_impl(cmdg_ctx *c, const double *Q, double *Qcopy, int write_
 
 template <int Nq>
 int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta, int mode,
-                         double *Qout, double rka, double bdt, int write_aux, const ElemSet &ES) {
+                         double *Qout, double rka, double bdt, int write_aux, int dq_store, const ElemSet &ES) {
     constexpr int NT = threads_for<Nq>();
     dim3 grid((unsigned)xcd_grid(c->affine ? ES.count : c->nelem)), block(NT);
     TimedLaunch tl(c, KIND_TENDENCY);
 #define CMDG_TEND_MIX(DIFF, ISO, MODE)                                                                           \
     hipLaunchKernelGGL((k_tendency_mix<Nq, DIFF, ISO, MODE>), grid, block, 0, c->stream, c->P, c->MP, c->AA, ES, Q, \
-                       c->d_gf32, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux)
+                       c->d_gf32, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux, dq_store)
 #define CMDG_TEND(DIFF, SMAG, MODE)                                                                               \
     do {                                                                                                          \
         if (c->mixed) {                                                                                           \
@@ -217,10 +217,10 @@ int launch_tendency_impl(cmdg_ctx *c, const double *Q, double *tend, double alph
         } else if (c->affine)                                                                                     \
             hipLaunchKernelGGL((k_tendency_aff<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->AA,    \
                                ES, Q, c->d_gf, c->d_f2tr, c->d_aux, tend, alpha, beta, Qout, rka, bdt,      \
-                               write_aux);                                                                        \
+                               write_aux, dq_store);                                                              \
         else                                                                                                      \
             hipLaunchKernelGGL((k_tendency<Nq, DIFF, SMAG, MODE>), grid, block, 0, c->stream, c->P, c->A, Q,      \
-                               c->d_gf, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux);                 \
+                               c->d_gf, c->d_aux, tend, alpha, beta, Qout, rka, bdt, write_aux, dq_store);       \
     } while (0)
     if (mode == 0) {
         if (!c->diffusive) CMDG_TEND(false, false, 0);
@@ -250,10 +250,13 @@ int launch_gradient(cmdg_ctx *c, const double *Q, double *Qcopy, int write_aux, 
 #undef CALL
 }
 
+// mode 1 (LSRK stage): beta 1 reads dQ, beta 0 takes it as zeros without reading it;
+// dq_store 0 leaves the stage's scaled dQ unstored (see k_tendency_mix)
 int launch_tendency(cmdg_ctx *c, const double *Q, double *tend, double alpha, double beta, int mode, double *Qout,
-                    double rka, double bdt, int write_aux, const ElemSet *ES = nullptr) {
+                    double rka, double bdt, int write_aux, int dq_store = 1, const ElemSet *ES = nullptr) {
     const ElemSet all{c->ord_tend, 0, c->nelem};
-#define CALL(NQ) launch_tendency_impl<NQ>(c, Q, tend, alpha, beta, mode, Qout, rka, bdt, write_aux, ES ? *ES : all)
+#define CALL(NQ)                                                                                                 \
+    launch_tendency_impl<NQ>(c, Q, tend, alpha, beta, mode, Qout, rka, bdt, write_aux, dq_store, ES ? *ES : all)
     CMDG_DISPATCH_NQ(CALL)
 #undef CALL
 }
@@ -945,11 +948,18 @@ int cmdg_lsrk54_steps(cmdg_ctx *c, double *Q, double *dQ, double t, double dt, i
             // step's write reaches the caller and the earlier steps skip it (theta_v costs
             // a power per node).
             const int wa = (s == 4 && step == nsteps - 1);
+            // The last stage scales dQ by RKA[0] = 0 and the next step's first stage adds
+            // the right-hand side to those zeros: between two steps of the call the zeros
+            // are neither stored nor read (dQ is read on entry and written on return as
+            // by nsteps dostep! calls; a zero in the state may differ in sign).
+            const double dq_read = (s == 0 && step > 0) ? 0.0 : 1.0;
+            const int dq_store = !(s == 4 && step < nsteps - 1);
             if (c->diffusive) {
                 const bool first = odd && step == 0 && s == 0 && !c->affine;
                 if ((rc = launch_gradient(c, first ? C : src, first ? S : nullptr, wa))) return rc;
             }
-            if ((rc = launch_tendency(c, src, dQ, 1.0, 1.0, 1, dst, a, bdt, c->diffusive ? 0 : wa))) return rc;
+            if ((rc = launch_tendency(c, src, dQ, 1.0, dq_read, 1, dst, a, bdt, c->diffusive ? 0 : wa, dq_store)))
+                return rc;
             double *tmp = src;
             src = dst;
             dst = tmp;

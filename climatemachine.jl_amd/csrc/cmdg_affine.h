@@ -339,13 +339,16 @@ __device__ __forceinline__ void face_flux_aff(const KParams &P, const AffArrays 
 
 // MODE 0: tend = alpha*RHS + beta*tend.   MODE 1: LSRK stage (tend is dQ):
 //   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka.  Qout is another buffer than Q:
-//   blocks read their neighbours' volume state.
+//   blocks read their neighbours' volume state.  MODE 1 takes alpha = 1 and beta = 1,
+//   or beta = 0 when dQ holds zeros (the previous stage scaled it by rka = 0): it is then
+//   not read; dq_store = 0 leaves the scaled dQ unstored (this stage's rka is 0 and the
+//   next stage is launched with beta = 0).
 template <int Nq, bool DIFF, bool SMAG, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>())
 k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
                const double *__restrict__ gfi, const double *__restrict__ f2tr, double *__restrict__ aux,
                double *__restrict__ tend, double alpha, double beta, double *__restrict__ Qout, double rka,
-               double bdt, int write_aux) {
+               double bdt, int write_aux, int dq_store) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NT = threads_for<Nq>();
     constexpr int NGF = SMAG ? 10 : 9;
     __shared__ double sD[Nq * Nq];
@@ -486,7 +489,7 @@ k_tendency_aff(KParams P, AffArrays A, ElemSet ES, const double *__restrict__ Q,
 #pragma unroll
             for (int s = 0; s < 5; ++s) {
                 qo[Np * s] = sQ[s][n] + bdt * T[s];
-                tb[Np * s] = T[s] * rka;
+                if (dq_store) tb[Np * s] = T[s] * rka;
             }
         }
     }

@@ -144,12 +144,14 @@ k_gradient(KParams P, KArrays A, const double *__restrict__ Q, double *__restric
 
 // ------------------------------------------------------------------ tendency kernel
 // MODE 0: tend = alpha*RHS + beta*tend.   MODE 1: LSRK stage, tend is dQ:
-//   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka.
+//   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka.  In MODE 1 beta = 0 says dQ holds
+//   zeros (the previous stage scaled it by rka = 0) and is not read; dq_store = 0 leaves
+//   the scaled dQ unstored (this stage's rka is 0 and the next stage gets beta = 0).
 template <int Nq, bool DIFF, bool SMAG, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>())
 k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__restrict__ gfi,
            double *__restrict__ aux, double *__restrict__ tend, double alpha, double beta,
-           double *__restrict__ Qout, double rka, double bdt, int write_aux) {
+           double *__restrict__ Qout, double rka, double bdt, int write_aux, int dq_store) {
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     constexpr int NGF = SMAG ? 10 : 9;
     __shared__ double sD[Nq * Nq];
@@ -220,7 +222,7 @@ k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__r
         }
         double out;
         if (MODE == 0) out = (beta != 0.0) ? alpha * th + beta * tb[Np * s] : alpha * th;
-        else out = th + tb[Np * s];
+        else out = (beta != 0.0) ? th + tb[Np * s] : th;
         double tv = 0.0;
 #pragma unroll
         for (int m = 0; m < Nq; ++m) {
@@ -302,7 +304,7 @@ k_tendency(KParams P, KArrays A, const double *__restrict__ Q, const double *__r
 #pragma unroll
         for (int s = 0; s < 5; ++s) {
             qo[Np * s] = q[s] + bdt * T[s];
-            tb[Np * s] = T[s] * rka;
+            if (dq_store) tb[Np * s] = T[s] * rka;
         }
     }
 }

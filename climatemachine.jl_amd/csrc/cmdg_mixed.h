@@ -458,7 +458,9 @@ __device__ __forceinline__ void face_flux_mix(const KParams &P, const MixParams 
 
 // MODE 0: tend = alpha*RHS + beta*tend.   MODE 1: LSRK stage (tend is dQ):
 //   dQ <- RHS + dQ; Qout = Qin + bdt*dQ; dQ *= rka.  Qout is another buffer than Q:
-//   blocks read their neighbours' volume state.
+//   blocks read their neighbours' volume state.  In MODE 1 beta = 0 says dQ holds zeros
+//   (the previous stage scaled it by rka = 0) and is not read; dq_store = 0 leaves the
+//   scaled dQ unstored (this stage's rka is 0 and the next stage gets beta = 0).
 // DIFF: second-order fluxes from the gradient flux gfi in stress form (ISO: its
 // isotropic layout); the closure (constant viscosity, Smagorinsky-Lilly or the learned
 // nu_0) is already folded into it.
@@ -468,7 +470,7 @@ template <int Nq, bool DIFF, bool ISO, int MODE>
 __global__ void __launch_bounds__(threads_for<Nq>()) __attribute__((amdgpu_waves_per_eu(Nq == 5 ? 6 : 1)))
 k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *__restrict__ Q,
                const float *__restrict__ gfi, double *__restrict__ aux, double *__restrict__ tend, double alpha,
-               double beta, double *__restrict__ Qout, double rka, double bdt, int write_aux) {
+               double beta, double *__restrict__ Qout, double rka, double bdt, int write_aux, int dq_store) {
 
     constexpr int Np = Nq * Nq * Nq, Nfp = Nq * Nq;
     constexpr int NT = threads_for<Nq>();
@@ -612,7 +614,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
     // the stage's register dQ, loaded before the barrier so that the wait overlaps it
     double dq[5] = {};
     double *tb = tend + (int64_t)Np * 5 * e + n;
-    if (MODE == 1 && act)
+    if (MODE == 1 && act && beta != 0.0)
 #pragma unroll
         for (int s = 0; s < 5; ++s) dq[s] = tb[Np * s];
     __syncthreads();
@@ -639,7 +641,7 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
             for (int s = 0; s < 5; ++s) {
                 const double dd = (double)T[s] + dq[s];
                 qo[Np * s] = sQ[s][n] + bdt * dd;
-                tb[Np * s] = dd * rka;
+                if (dq_store) tb[Np * s] = dd * rka;
             }
         }
     }

@@ -154,10 +154,13 @@ int cmdg_lsrk54_step(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt);
 /* nsteps consecutive LSRK54CarpenterKennedy steps from t with step dt: the loop of
  * solve!(Q, solver; numberofsteps) with no callbacks, src/Numerics/ODESolvers/
  * ODESolvers.jl:110-158 over dostep! (LowStorageRungeKuttaMethod.jl:102-144).
- * Identical to nsteps calls of cmdg_lsrk54_step.  The stages alternate the state
+ * Identical to nsteps calls of cmdg_lsrk54_step (Q, dQ and the auxiliary theta_v and
+ * air_T on return; a zero may differ in sign).  The stages alternate the state
  * between Q and a context buffer (each stage reads its neighbours' state while it
  * writes the next); the result is in Q on return, and a call with an odd nsteps
- * copies Q into the context buffer once at its start. */
+ * copies Q into the context buffer once at its start.  Between two steps of the call
+ * the zeros the last stage leaves in dQ (its coefficient RKA[1] is 0) are neither
+ * stored nor read, and only the last step writes theta_v and air_T. */
 int cmdg_lsrk54_steps(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt, int64_t nsteps);
 
 /* One low-storage stage: dQ <- RHS(Q, t_stage) + dQ; Q += rkb*dt*dQ;

@@ -165,7 +165,19 @@ def test_lsrk54_step_parity(name, geometry):
     got2 = to_host(Q2)
     for s in range(5):
         assert rel_l2(got2[:, s] - Q0[:, s], Qr[:, s] - Q0[:, s]) < 1e-11
-    # aux after a step holds the last stage's theta_v / air_T, as after dostep!
+    # aux after the steps holds the last stage's theta_v / air_T, as after dostep!, and
+    # dQ the last stage's zeros (the staged path above rewrote aux: two more steps in one
+    # call from the oracle's state, against the oracle's fourth and fifth steps)
+    Q3 = to_device(Qr)
+    solver3 = LSRK54CarpenterKennedy(dg, Q3, dt=dt, t0=3 * dt)
+    solve(Q3, solver3, numberofsteps=2, timeend=math.inf)
+    for k in (3, 4):
+        orc.lsrk54_step(Qr, dQr, k * dt, dt)
+    aux_dev = to_host(dg.state_auxiliary)
+    for key in ("theta_v", "air_T"):
+        assert rel_l2(aux_dev[:, cfg.layout[key]], orc.aux[:, cfg.layout[key]]) < RTOL, key
+    assert np.array_equal(to_host(solver3.dQ), dQr)
+    assert not dQr.any()
     dg.close()
 
 
@@ -588,10 +600,12 @@ def test_traversal_orders_agree(precision):
 @pytest.mark.parametrize("variant", ["fp64_affine", "fp64_general", "mixed", "euler_affine"])
 def test_multistep_matches_single_steps(variant):
     """cmdg_lsrk54_steps (the solve! loop without callbacks; the stages alternate the
-    state between two buffers across the steps of the call) gives bit for bit the state
-    of the same number of single dostep! calls, and the same auxiliary theta_v and air_T
-    (those of the last stage's state, written once, by the call's last step: they are
-    cleared to NaN before the call, so the check sees that write)."""
+    state between two buffers across the steps of the call, and the zeros the last stage
+    leaves in dQ are neither stored nor read between two steps) gives the state and dQ of
+    the same number of single dostep! calls (IEEE equality: a zero may differ in sign),
+    and the same auxiliary theta_v and air_T (those of the last stage's state, written
+    once, by the call's last step: they are cleared to NaN before the call, so the check
+    sees that write)."""
     import os
     import torch
     from cmdg.dg import DGModel, LSRK54CarpenterKennedy, solve, to_device, to_host
@@ -623,6 +637,7 @@ def test_multistep_matches_single_steps(variant):
     assert np.isfinite(a).all()
     assert not np.array_equal(a, Q0)
     assert np.array_equal(a, b)
+    assert np.array_equal(to_host(sa.dQ), to_host(sb.dQ))
     aux_b = to_host(dg.state_auxiliary)
     for f in outs:
         assert np.isfinite(aux_b[:, f, :]).all(), f
