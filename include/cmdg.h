@@ -79,8 +79,9 @@ typedef struct cmdg_desc {
     int32_t aux_ref_rho;    /* ref_state.rho */
     int32_t aux_ref_p;      /* ref_state.p */
     int32_t aux_Delta;      /* turbulence.Delta (Smagorinsky length scale) */
-    int32_t aux_theta_v;    /* moisture.theta_v  (written by every RHS) */
-    int32_t aux_air_T;      /* moisture.air_T    (written by every RHS) */
+    int32_t aux_theta_v;    /* moisture.theta_v  (written by every RHS; by the last
+                             * step of a cmdg_lsrk54_steps call) */
+    int32_t aux_air_T;      /* moisture.air_T    (the same) */
     const double *vgeo;
     const double *sgeo;
     const int64_t *vmap_minus;
@@ -159,8 +160,9 @@ int cmdg_lsrk54_step(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt);
  * between Q and a context buffer (each stage reads its neighbours' state while it
  * writes the next); the result is in Q on return, and a call with an odd nsteps
  * copies Q into the context buffer once at its start.  Between two steps of the call
- * the zeros the last stage leaves in dQ (its coefficient RKA[1] is 0) are neither
- * stored nor read, and only the last step writes theta_v and air_T. */
+ * the zeros the last stage leaves in dQ (the scheme's first coefficient, A_1 = 0,
+ * scales it) are neither stored nor read, and only the last step writes theta_v and
+ * air_T. */
 int cmdg_lsrk54_steps(cmdg_ctx *ctx, double *Q, double *dQ, double t, double dt, int64_t nsteps);
 
 /* One low-storage stage: dQ <- RHS(Q, t_stage) + dQ; Q += rkb*dt*dQ;
