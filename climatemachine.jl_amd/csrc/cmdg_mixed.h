@@ -216,7 +216,8 @@ __device__ __forceinline__ void grad_arg_f(const KParams &P, const MixParams &M,
 // the lifted jumps run a field at a time and the stress form is stored as it is formed,
 // which keeps the learned variants at 66-70 registers and the Smagorinsky-Lilly ones at
 // 49-57; a whole-gradient-then-lifting order needed 79-94 (5-6 waves) and spilled under an
-// 80 cap with a gravitational potential.  Measured on one box (whole order -> field
+// 80 cap with a gravitational potential.  Eight waves (a cap of 64, which the LDS would
+// allow the learned block) spill 9 registers without and 20 with a potential.  Measured on one box (whole order -> field
 // order): 96^3 chosen form 56.8 -> 57.1 GDOF/s (gradient 4036 -> 3975 us), mixed
 // Smagorinsky-Lilly 60.6 -> 61.9; 25^3 50.1 -> 50.4 (86.7 -> 84.5 us).  Without a
 // gravitational potential the kernel also skips N^2, Ri and the buoyancy factor (exactly
