@@ -612,12 +612,16 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 #pragma unroll
         for (int s = 0; s < 5; ++s) sFc[t][s] = fl5[s];
     }
-    // the stage's register dQ, loaded before the barrier so that the wait overlaps it
+    // the stage's register dQ, loaded before the barrier so that the wait overlaps it.
+    // dQ and the stage's outputs Q' and dQ' pass through once (the next reader comes a
+    // whole grid later), so they go nontemporal and leave the cache to the face
+    // neighbours' lines (96^3: 5590 -> 5513 us a launch; nontemporal stress-form stores
+    // in the gradient kernel cost it 3815 -> 3924 us and were left out)
     double dq[5] = {};
     double *tb = tend + (int64_t)Np * 5 * e + n;
     if (MODE == 1 && act && beta != 0.0)
 #pragma unroll
-        for (int s = 0; s < 5; ++s) dq[s] = tb[Np * s];
+        for (int s = 0; s < 5; ++s) dq[s] = __builtin_nontemporal_load(tb + Np * s);
     __syncthreads();
 
     if (act) {
@@ -641,8 +645,8 @@ k_tendency_mix(KParams P, MixParams MP, AffArrays A, ElemSet ES, const double *_
 #pragma unroll
             for (int s = 0; s < 5; ++s) {
                 const double dd = (double)T[s] + dq[s];
-                qo[Np * s] = sQ[s][n] + bdt * dd;
-                if (dq_store) tb[Np * s] = dd * rka;
+                __builtin_nontemporal_store(sQ[s][n] + bdt * dd, qo + Np * s);
+                if (dq_store) __builtin_nontemporal_store(dd * rka, tb + Np * s);
             }
         }
     }
